@@ -1,0 +1,186 @@
+"""Benchmark of the MI355X gridrec path: plain FP32 gridrec of a batch of slices.
+
+Workload (one "step" = one pass of gridrec over the whole batch):
+  2048 slices, each 1500 angles x 2048 detectors float32 (25.2 GB resident in
+  HBM), seeded 20-ellipse phantoms projected analytically, Poisson noise at
+  1e4 photons per cell and -log; rotation centre 1024 + U[-3,3] for the
+  dataset; each row half-padded to 4096 (edge replication, generated on chip),
+  PSWF C = 6, Shepp-Logan filter; output the 2048 x 2048 window of each
+  4095 x 4095 image (slices processed in pairs, two per complex transform).
+
+Prints one JSON line: slices/s over the timed steps, per-kernel roofline of the
+dominant kernel from HIP events recorded around every launch in the timed
+region, accuracy of one pair against the reference's gridrec (its FP32
+restatement, and the reference build when present) and the FP64 oracle, and
+the oracle's CPU throughput on a bounded sample on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "slices/s @2048² (1500 ang) plain vs batched vs learned; rel-L2 vs FP64, PSNR"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--slices", type=int, default=2048)
+    ap.add_argument("--n-ang", type=int, default=1500)
+    ap.add_argument("--n-det", type=int, default=2048)
+    ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--accum", default="compensated", choices=["compensated", "plain"])
+    ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-accuracy", action="store_true")
+    return ap.parse_args()
+
+
+def kernel_bytes(kind, n_ang, n_det, half, M, out_n, pairs):
+    """Algorithmic HBM bytes of one launch covering `pairs` slice pairs:
+    every input read once, every output written once."""
+    spec = 2 * n_ang * half * 8          # Cdata1 + Cdata2, complex64
+    if kind == "rowfft":
+        b = 2 * n_ang * n_det * 4 + spec  # two unpadded float rows in, spectra out
+    elif kind == "grid":
+        b = spec + M * M * 8              # spectra in, complex grid out
+    elif kind == "ifft_rows":
+        b = M * M * 8 + M * out_n * 8     # grid in, pruned rows out
+    else:
+        b = M * out_n * 8 + 2 * out_n * out_n * 4  # pruned rows in, two float images out
+    return b * pairs
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(slice pairs are independent: several GPUs would each take a share of the batch)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import oracle as O
+    from projects_amd import gridrec as gr
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    ns, nang, ndet = a.slices, a.n_ang, a.n_det
+    rng = np.random.default_rng(a.seed)
+    center = float(np.float32(ndet / 2 + rng.uniform(-3.0, 3.0)))
+
+    # synthetic sinograms, generated in HBM
+    sino = torch.empty((ns, nang, ndet), dtype=torch.float32, device=dev)
+    t0 = time.time()
+    chunk = 256
+    for s0 in range(0, ns, chunk):
+        k = min(chunk, ns - s0)
+        gr.phantom_sinograms(sino[s0:s0 + k], a.seed, nang, ndet, [center] * k, 20, photons=1e4, slice0=s0)
+    torch.cuda.synchronize()
+    gen_s = time.time() - t0
+
+    plan = gr.Plan(n_ang=nang, n_det=ndet, center=center, pad="half", crop=(ndet // 2, ndet), accum=a.accum)
+    inf = plan.info
+    out_n = plan.out_n
+    out = torch.empty((ns, out_n, out_n), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(a.warmup):
+        plan.recon(sino, out)
+    torch.cuda.synchronize()
+    plan.set_timing(True)
+    plan.timing(reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        plan.recon(sino, out, stream=stream)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    tim = plan.timing(reset=True)
+    plan.set_timing(False)
+    slices_s = ns * a.steps / dt
+
+    # roofline of the dominant kernel (events recorded in the timed region)
+    half, M = inf["pdim"] // 2, inf["M"]
+    pairs_total = tim["pairs"]
+    kinds = {"rowfft": ("rowfft_ms", "rowfft_launches"), "grid": ("grid_ms", "grid_launches"),
+             "ifft_rows": ("ifft_rows_ms", "ifft_rows_launches"), "ifft_cols": ("ifft_cols_ms", "ifft_cols_launches")}
+    per_kernel = {}
+    for kname, (ms_f, n_f) in kinds.items():
+        launches = max(1, tim[n_f])
+        avg_ms = tim[ms_f] / launches
+        pairs_per_launch = pairs_total / launches
+        b = kernel_bytes(kname, nang, ndet, half, M, out_n, pairs_per_launch)
+        per_kernel[kname] = dict(total_ms=round(tim[ms_f], 3), launches=tim[n_f], avg_ms=round(avg_ms, 4),
+                                 alg_bytes=int(b), gbs=round(b / (avg_ms * 1e-3) / 1e9, 1))
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
+    d = per_kernel[dom]
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["gbs"] / HBM_PEAK_GBS, 5), "traffic": None}
+    # whole-pipeline algorithmic bytes per pair (8 n_ang n_det + 16 n_ang pdim + 32 M^2 + 8 n_out^2)
+    b_pair = 8 * nang * ndet + 16 * nang * inf["pdim"] + 32 * M * M + 8 * out_n * out_n
+    pipeline_gbs = b_pair * (slices_s / 2) / 1e9
+
+    result = {
+        "metric": METRIC, "value": round(slices_s, 2), "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(dt / a.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded 20-ellipse phantoms, analytic Radon, Poisson(1e4)+log",
+        "config": {"workload": "plain gridrec, 2048 slices x 1500 angles x 2048 det, half-pad to 4096, "
+                               "PSWF C=6, shlo, 2048^2 window of 4095^2",
+                   "slices": ns, "n_ang": nang, "n_det": ndet, "center": center, "pdim": inf["pdim"], "M": M,
+                   "M0": inf["M0"], "out_n": out_n, "accum": a.accum, "tables": "reference",
+                   "pairs_per_launch": inf["max_pairs"]},
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "pipeline_alg_GBs": round(pipeline_gbs, 1),
+        "gen_s": round(gen_s, 2),
+    }
+
+    # accuracy on pair 0 (the timed output)
+    if not a.no_accuracy:
+        s = sino[0:2].cpu().numpy()
+        o = out[0:2].cpu().numpy()
+        g = O.Geometry(n_ang=nang, n_det=ndet, center=center).padded()
+        p1, p2 = O.pad_half(s[0]), O.pad_half(s[1])
+        c = slice(ndet // 2, ndet // 2 + out_n)
+        r1, r2 = O.recon_f32(g, p1, p2)
+        d1, d2 = O.recon_f64(g, p1, p2)
+        acc = {"rel_l2_vs_reference_restatement": max(O.rel_l2(o[0], r1[c, c]), O.rel_l2(o[1], r2[c, c])),
+               "rel_l2_vs_fp64": max(O.rel_l2(o[0], d1[c, c]), O.rel_l2(o[1], d2[c, c])),
+               "max_abs_vs_fp64": float(max(np.abs(o[0] - d1[c, c]).max(), np.abs(o[1] - d2[c, c]).max())),
+               "reference_restatement_rel_l2_vs_fp64": max(O.rel_l2(r1[c, c], d1[c, c]), O.rel_l2(r2[c, c], d2[c, c]))}
+        if O.ref_available():
+            q1, q2 = O.ref_recon(g, p1, p2)
+            acc["rel_l2_vs_reference_build"] = max(O.rel_l2(o[0], q1[c, c]), O.rel_l2(o[1], q2[c, c]))
+        result["accuracy"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
+
+    # CPU baseline: the oracle's FP32 restatement on this host's cores, bounded sample
+    if not a.no_cpu:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        pairs = a.cpu_pairs or threads
+        pairs = min(pairs, ns // 2)
+        s = sino[0:2 * pairs].cpu().numpy()
+        g = O.Geometry(n_ang=nang, n_det=ndet, center=center)
+        t0 = time.perf_counter()
+        O.recon_batch_f32(g, s, True, ndet // 2, out_n, threads=threads)
+        cdt = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(2 * pairs / cdt, 3), "unit": "slices/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"{2 * pairs} slices ({pairs} pairs) of the same workload, FP32 "
+                                            f"restatement of GridrecCPP, {threads} threads, {cdt:.1f} s"}
+    print(json.dumps(result), flush=True)
+    plan.close()
+
+
+if __name__ == "__main__":
+    main()

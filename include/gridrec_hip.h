@@ -1,0 +1,185 @@
+/*
+ * gridrec_hip.h -- C-ABI of the MI355X gridrec library (libgridrec_hip.so).
+ *
+ * Plain C types only: pointers, sizes, status codes.  A HIP stream is passed
+ * as an opaque `void *` (a hipStream_t; NULL = the plan's own stream).
+ *
+ * Each entry point names the reference interface it replaces
+ * (gridrec/GridrecCPP/Gridrec5.2 unless stated):
+ *
+ *   gr_get_pswf      <- get_pswf          pswf.cpp:36-47, grid.h:83
+ *   gr_filter_lookup <- get_filter        filters.cpp:36-48, grid.h:86
+ *   gr_plan_create   <- recon_init        grid.cpp:54-147, grid.h:89
+ *   gr_recon_host    <- do_recon          grid.cpp:152-415 (5.x globals form),
+ *                                         Gridrec4.9/grid.cpp:146 (explicit form)
+ *   gr_recon_device  <- do_recon over a batch of slice pairs already in HBM
+ *                       (the pair loop of Multitomo_unit.txt:823-856 and
+ *                       tomo_recon.cpp:3700-3863 moved onto the GPU)
+ *   gr_plan_destroy  <- recon_free        grid.cpp:418-434
+ *
+ * All functions return 0 on success and a negative GR_E* code on failure;
+ * gr_last_error() returns the message of the calling thread's last failure.
+ * Unlike the reference (exit(2) in trig_su/legendre/get_pswf) nothing here
+ * terminates the process.
+ */
+#ifndef GRIDREC_HIP_H
+#define GRIDREC_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+enum {
+    GR_OK = 0,
+    GR_EINVAL = -1,     /* invalid argument or geometry */
+    GR_ENOMEM = -2,     /* device or host allocation failed */
+    GR_EHIP = -3,       /* a HIP runtime call failed */
+    GR_ENOPSWF = -4,    /* no PSWF for this C (pswf.cpp:39-44) */
+    GR_ENODEV = -5      /* no usable GPU */
+};
+
+/* Padding applied to each sinogram row before the row FFT. */
+enum {
+    GR_PAD_NONE = 0,       /* the reference: rows of n_det, zero-filled to pdim (grid.cpp:240-252) */
+    GR_PAD_HALF_EDGE = 1   /* tomo_recon.cpp:3691-3807 with xoffset 0: width 2*n_det, first/last
+                              value replicated into n_det/2 cells each side, centre + n_det/2 */
+};
+
+/* Accumulation of the gridding taps into each H cell. */
+enum {
+    GR_ACCUM_COMPENSATED = 0,  /* error-free 2Sum per tap, sum + compensation at the end */
+    GR_ACCUM_PLAIN = 1         /* plain float adds, as grid.cpp:297-300 */
+};
+
+/* Where the filter/PSWF/deapodization tables come from. */
+enum {
+    GR_TABLES_REFERENCE = 0,   /* bit-identical to recon_init's float tables (host) */
+    GR_TABLES_FP64 = 1         /* evaluated in FP64 on the GPU, then rounded to float */
+};
+
+/* PSWF record, layout of pswf_struct (grid.h:37-44). */
+typedef struct gr_pswf {
+    float C;
+    int nt;
+    float lmbda;
+    float coefs[15];
+} gr_pswf;
+
+typedef struct gr_params {
+    /* sg_struct (grid.h:26-35) */
+    int n_ang;             /* angles per sinogram */
+    int n_det;             /* detector cells per row of the input */
+    int geom;              /* 0: angle list in degrees; 1: uniform on [0,pi); 2: on [0,2pi) */
+    const float *angles;   /* host array of n_ang degrees when geom == 0 (copied at create) */
+    float center;          /* rotation axis, 0-based detector coordinate of the input rows */
+    /* grid_struct (grid.h:46-58) */
+    float C;               /* PSWF parameter: 4, 4.2, 5, 6 or 7 */
+    float sampl;           /* oversampling ratio (the plan reports the adjusted M/M0) */
+    float MaxPixSiz;
+    float R;
+    float X0, Y0;          /* ROI offset (units of centre-to-edge distance) */
+    char filter[16];       /* shlo|shepp|hann|hamm|hamming|ramp|ramlak; unknown -> shlo */
+    long ltbl;             /* PSWF lookup table length (reference default 512) */
+    /* extensions of this library */
+    int pad;               /* GR_PAD_* */
+    long crop_off;         /* output window origin in the M0 x M0 image, rows and columns */
+    long crop_n;           /* output window size; 0 means the whole M0 x M0 image */
+    int accum;             /* GR_ACCUM_* */
+    int tables;            /* GR_TABLES_* */
+    int device;            /* HIP device ordinal */
+    int max_pairs;         /* slice pairs per launch (workspace sizing); 0 = automatic */
+    /* optional overrides, as recon_init honours grid_struct.filter and
+       grid_struct.pswf (grid.cpp:78-81): a filter function evaluated on the
+       host for the filter table (only with GR_TABLES_REFERENCE), and a PSWF
+       record used instead of the database entry for C.  NULL = by name / C. */
+    float (*filter_fn)(float);
+    const gr_pswf *pswf;
+} gr_params;
+
+typedef struct gr_info {
+    long pdim, M, M0, M02;  /* sizes of recon_init (grid.cpp:87-113) */
+    float sampl;            /* adjusted sampl = M/M0 (grid.cpp:117) */
+    float L, scale;         /* grid.cpp:119-120 */
+    int n_det_eff;          /* row width seen by gridrec after padding */
+    float center_eff;       /* centre seen by gridrec after padding */
+    long out_off, out_n;    /* output window */
+    int max_pairs;
+    size_t device_bytes;    /* device memory held by the plan */
+} gr_info;
+
+/* Per-kernel timing of the last gr_recon_device call(s) (ms, summed over
+ * launches) when timing is enabled. */
+typedef struct gr_timing {
+    double rowfft_ms, grid_ms, ifft_rows_ms, ifft_cols_ms;
+    long rowfft_launches, grid_launches, ifft_rows_launches, ifft_cols_launches;
+    long pairs;
+} gr_timing;
+
+typedef struct gr_plan gr_plan;
+
+/* get_pswf: copy of the database record whose C is within 0.01 of C. */
+int gr_get_pswf(float C, gr_pswf *out);
+
+/* get_filter: canonical filter id (0 shlo, 1 hann, 2 hamm, 3 ramp) for name;
+ * an unknown name selects shlo and, as the reference does, writes "shlo" into
+ * name_io (which must hold 16 bytes) -- pass NULL to skip the write-back. */
+int gr_filter_lookup(const char *name, char *name_io, int *filter_id);
+
+/* recon_init: sizes, device tables and workspace for one geometry. */
+int gr_plan_create(const gr_params *params, gr_plan **plan);
+int gr_plan_info(const gr_plan *plan, gr_info *info);
+
+/* Copies of the plan's device tables (the globals SINE, COSE, filphase, wtbl,
+ * winv of grid.cpp:32-33): n_ang, n_ang, pdim (pdim/2 complex), ltbl+1 and M0
+ * floats.  Any pointer may be NULL. */
+int gr_plan_tables(const gr_plan *plan, float *sine, float *cose, float *filphase, float *wtbl,
+                   float *winv);
+
+/* do_recon on host rows: G1/G2 are n_ang row pointers of n_det floats (any
+ * stride), S1/S2 are out_n row pointers of out_n floats.  G2 may equal G1.
+ * Synchronous. */
+int gr_recon_host(gr_plan *plan, const float *const *G1, const float *const *G2,
+                  float *const *S1, float *const *S2);
+
+/* do_recon for n_slices slices resident in device memory.  Slice s starts at
+ * sino + s*sino_slice_stride and holds n_ang rows of n_det floats spaced
+ * sino_row_stride floats apart.  Slices (2p, 2p+1) form pair p and share one
+ * complex transform (grid.cpp:240-246); an odd last slice is paired with zeros.
+ * Image s is written to out + s*out_slice_stride as out_n rows of out_n floats
+ * spaced out_row_stride apart.  Asynchronous on `stream`. */
+int gr_recon_device(gr_plan *plan, const float *sino, long sino_slice_stride,
+                    long sino_row_stride, float *out, long out_slice_stride,
+                    long out_row_stride, int n_slices, void *stream);
+
+/* recon_free. */
+int gr_plan_destroy(gr_plan *plan);
+
+/* Per-kernel HIP-event timing of gr_recon_device (adds a sync per launch). */
+int gr_set_timing(gr_plan *plan, int enable);
+int gr_get_timing(gr_plan *plan, gr_timing *out, int reset);
+
+/* Synthetic sinograms on the device (the generator of projects_amd/phantom.py):
+ * n_slices slices [slice0, slice0+n_slices) of seed, each n_ang x n_det floats,
+ * n_ell ellipses; photons > 0 adds Poisson noise and -log; density 0 is
+ * U[-0.3,1], 1 is log-uniform over 4 decades.  centers[s] is the axis of slice
+ * s (n_slices floats, host). */
+int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0,
+                         unsigned long long seed, int n_ang, int n_det,
+                         const float *centers, int n_ell, float photons, int density,
+                         void *stream);
+
+const char *gr_last_error(void);
+const char *gr_version(void);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,479 @@
+// gr_capi.cpp -- the C-ABI of include/gridrec_hip.h.
+//
+// A plan is what recon_init sets up (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:54-147):
+// sizes, the sine/cosine, filter-phase, PSWF and deapodization tables -- here
+// resident in HBM -- plus this library's own device tables (sorted angle keys
+// for the gather, the tile order, FFT twiddles) and a workspace for
+// `max_pairs` slice pairs.  gr_recon_device runs K1..K4 over a batch of pairs
+// in chunks of max_pairs; nothing is allocated per call.  Plans are independent
+// objects: unlike the reference's globals, any number may coexist.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "gr_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return GR_EHIP;
+}
+
+#define GR_HIP(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+    } while (0)
+
+}  // namespace
+
+struct gr_plan {
+    gr_params prm;
+    GrGeom g;
+    std::vector<float> angles;  // copy of the caller's list (geom 0)
+    hipStream_t stream = nullptr;
+    // device tables
+    float *d_cose = nullptr, *d_sine = nullptr, *d_roi = nullptr, *d_wtbl = nullptr, *d_winv = nullptr;
+    float2 *d_filphase = nullptr, *d_tw_row = nullptr, *d_tw_grid = nullptr;
+    float *d_akey = nullptr;
+    int *d_aord = nullptr;
+    unsigned *d_tiles = nullptr;
+    int ntiles = 0;
+    // workspace
+    int max_pairs = 1;
+    float2 *d_spec1 = nullptr, *d_spec2 = nullptr, *d_ht = nullptr, *d_z = nullptr;
+    size_t dev_bytes = 0;
+    // host staging for gr_recon_host
+    float *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr;
+    // timing: event pairs recorded around launches, resolved in gr_get_timing
+    int timing = 0;
+    gr_timing t{};
+    struct Pending {
+        int kind;  // 0 rowfft, 1 grid, 2 ifft rows, 3 ifft cols
+        hipEvent_t e0, e1;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    GrDevTables tables() const {
+        GrDevTables t;
+        t.cose = d_cose;
+        t.sine = d_sine;
+        t.roi_offset = d_roi;
+        t.filphase = d_filphase;
+        t.wtbl = d_wtbl;
+        t.winv = d_winv;
+        t.tw_row = d_tw_row;
+        t.tw_grid = d_tw_grid;
+        t.akey = d_akey;
+        t.aord = d_aord;
+        t.tiles = d_tiles;
+        t.ntiles = ntiles;
+        return t;
+    }
+};
+
+namespace {
+
+template <typename T>
+int dev_upload(gr_plan *pl, T **dst, const T *src, size_t n) {
+    GR_HIP(hipMalloc((void **)dst, sizeof(T) * (n ? n : 1)));
+    pl->dev_bytes += sizeof(T) * (n ? n : 1);
+    if (n) GR_HIP(hipMemcpy(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice));
+    return GR_OK;
+}
+
+template <typename T>
+int dev_alloc(gr_plan *pl, T **dst, size_t n) {
+    hipError_t e = hipMalloc((void **)dst, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) return fail(GR_ENOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
+    pl->dev_bytes += sizeof(T) * (n ? n : 1);
+    return GR_OK;
+}
+
+void free_plan(gr_plan *pl) {
+    if (!pl) return;
+    void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase,
+                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_tiles, pl->d_spec1,
+                    pl->d_spec2, pl->d_ht, pl->d_z, pl->d_in, pl->d_out};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (pl->h_in) (void)hipHostFree(pl->h_in);
+    if (pl->h_out) (void)hipHostFree(pl->h_out);
+    for (auto &q : pl->pending) {
+        (void)hipEventDestroy(q.e0);
+        (void)hipEventDestroy(q.e1);
+    }
+    for (hipEvent_t e : pl->event_pool) (void)hipEventDestroy(e);
+    if (pl->stream) (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+std::vector<float2> twiddles(long n, int sign) {
+    std::vector<float2> t(n);
+    for (long m = 0; m < n; m++) {
+        const double a = 2.0 * M_PI * (double)m / (double)n;
+        t[m] = make_float2((float)cos(a), (float)(sign * sin(a)));
+    }
+    return t;
+}
+
+hipEvent_t pool_event(gr_plan *pl) {
+    if (!pl->event_pool.empty()) {
+        hipEvent_t e = pl->event_pool.back();
+        pl->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+// Per-launch timing: an event pair around one kernel on its own stream; no
+// synchronisation here, the pairs are resolved by gr_get_timing.
+struct KernelTimer {
+    gr_plan *pl;
+    hipStream_t s;
+    int kind;
+    hipEvent_t e0 = nullptr;
+    KernelTimer(gr_plan *p, hipStream_t st, int k) : pl(p), s(st), kind(k) {
+        if (pl->timing) {
+            e0 = pool_event(pl);
+            (void)hipEventRecord(e0, s);
+        }
+    }
+    void done() {
+        if (!pl->timing) return;
+        hipEvent_t e1 = pool_event(pl);
+        (void)hipEventRecord(e1, s);
+        pl->pending.push_back({kind, e0, e1});
+    }
+};
+
+void resolve_timing(gr_plan *pl) {
+    for (auto &q : pl->pending) {
+        (void)hipEventSynchronize(q.e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, q.e0, q.e1);
+        switch (q.kind) {
+        case 0: pl->t.rowfft_ms += ms; pl->t.rowfft_launches++; break;
+        case 1: pl->t.grid_ms += ms; pl->t.grid_launches++; break;
+        case 2: pl->t.ifft_rows_ms += ms; pl->t.ifft_rows_launches++; break;
+        default: pl->t.ifft_cols_ms += ms; pl->t.ifft_cols_launches++; break;
+        }
+        pl->event_pool.push_back(q.e0);
+        pl->event_pool.push_back(q.e1);
+    }
+    pl->pending.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gr_last_error(void) { return g_err.c_str(); }
+const char *gr_version(void) { return "gridrec-mi355x 0.1 (gfx950)"; }
+
+int gr_get_pswf(float C, gr_pswf *out) {
+    if (!out) return fail(GR_EINVAL, "null output");
+    if (gr_find_pswf(C, out)) {
+        char buf[96];
+        snprintf(buf, sizeof buf, "Prolate parameter, C = %f not in data base", C);
+        return fail(GR_ENOPSWF, buf);
+    }
+    return GR_OK;
+}
+
+int gr_filter_lookup(const char *name, char *name_io, int *filter_id) {
+    int id = gr_filter_id_of(name);
+    if (id < 0) {
+        id = 0;  // filters.cpp:43-47: default shlo, name rewritten
+        if (name_io) strcpy(name_io, "shlo");
+    }
+    if (filter_id) *filter_id = id;
+    return GR_OK;
+}
+
+int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
+    if (!plan_out) return fail(GR_EINVAL, "null plan pointer");
+    *plan_out = nullptr;
+    const char *why = "";
+    gr_plan *pl = new gr_plan();
+    pl->prm = *params;
+    int rc = gr_geom_from_params(params, &pl->g, &why);
+    if (rc) {
+        delete pl;
+        return fail(rc, why);
+    }
+    const GrGeom &g = pl->g;
+    const int n_ang = params->n_ang;
+    if (params->geom == 0) {
+        pl->angles.assign(params->angles, params->angles + n_ang);
+        pl->prm.angles = pl->angles.data();
+    } else {
+        pl->prm.angles = nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        delete pl;
+        return fail(GR_ENODEV, "no HIP device");
+    }
+    if (params->device < 0 || params->device >= ndev) {
+        delete pl;
+        return fail(GR_EINVAL, "device ordinal out of range");
+    }
+#define CK(x)                  \
+    do {                       \
+        rc = (x);              \
+        if (rc) {              \
+            free_plan(pl);     \
+            return rc;         \
+        }                      \
+    } while (0)
+    CK(hipSetDevice(params->device) == hipSuccess ? 0 : fail(GR_EHIP, "hipSetDevice failed"));
+    CK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess ? 0 : fail(GR_EHIP, "stream"));
+
+    // reference tables (host, bit-identical to recon_init)
+    std::vector<float> sine(n_ang), cose(n_ang), roi(n_ang), filphase(2 * g.half), wtbl(params->ltbl + 1),
+        winv(g.M0);
+    gr_host_tables(&pl->prm, g, sine.data(), cose.data(), filphase.data(), wtbl.data(), winv.data(), roi.data());
+    CK(dev_upload(pl, &pl->d_sine, sine.data(), n_ang));
+    CK(dev_upload(pl, &pl->d_cose, cose.data(), n_ang));
+    CK(dev_upload(pl, &pl->d_roi, roi.data(), n_ang));
+    CK(dev_upload(pl, &pl->d_filphase, (const float2 *)filphase.data(), g.half));
+    CK(dev_upload(pl, &pl->d_wtbl, wtbl.data(), wtbl.size()));
+    CK(dev_upload(pl, &pl->d_winv, winv.data(), winv.size()));
+    if (params->tables == GR_TABLES_FP64) {
+        hipError_t e = gr_launch_tables_fp64(g, &pl->prm, pl->d_filphase, pl->d_wtbl, pl->d_winv, pl->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+        if (e != hipSuccess) {
+            free_plan(pl);
+            return hip_fail(e, "FP64 tables");
+        }
+    }
+    // twiddles
+    std::vector<float2> tr = twiddles(g.pdim, +1), tg = twiddles(g.M, -1);
+    CK(dev_upload(pl, &pl->d_tw_row, tr.data(), tr.size()));
+    CK(dev_upload(pl, &pl->d_tw_grid, tg.data(), tg.size()));
+    // angle keys: direction of each sample line, atan2(SINE, COSE) mod pi, sorted
+    {
+        std::vector<double> key(n_ang);
+        for (int n = 0; n < n_ang; n++) {
+            double k = atan2((double)sine[n], (double)cose[n]);
+            if (k < 0) k += M_PI;
+            if (k >= M_PI) k -= M_PI;
+            key[n] = k;
+        }
+        std::vector<int> ord(n_ang);
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return key[x] < key[y]; });
+        std::vector<float> akey(n_ang);
+        for (int i = 0; i < n_ang; i++) akey[i] = (float)key[ord[i]];
+        // keep the float keys sorted after rounding
+        for (int i = 1; i < n_ang; i++) akey[i] = std::max(akey[i], akey[i - 1]);
+        CK(dev_upload(pl, &pl->d_akey, akey.data(), n_ang));
+        CK(dev_upload(pl, &pl->d_aord, ord.data(), n_ang));
+    }
+    // 8x8 tiles of the M x M grid, nearest to the centre first
+    {
+        const int nt = (int)((g.M + 7) / 8);
+        const double c = (double)(g.M >> 1);
+        std::vector<unsigned> tiles;
+        std::vector<double> dist;
+        tiles.reserve((size_t)nt * nt);
+        for (int ty = 0; ty < nt; ty++)
+            for (int tx = 0; tx < nt; tx++) tiles.push_back((unsigned)tx | ((unsigned)ty << 16));
+        dist.resize(tiles.size());
+        for (size_t i = 0; i < tiles.size(); i++) {
+            const double x = (double)(tiles[i] & 0xffff) * 8 + 3.5 - c, y = (double)(tiles[i] >> 16) * 8 + 3.5 - c;
+            dist[i] = x * x + y * y;
+        }
+        std::vector<size_t> idx(tiles.size());
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return dist[x] < dist[y]; });
+        std::vector<unsigned> sorted(tiles.size());
+        for (size_t i = 0; i < idx.size(); i++) sorted[i] = tiles[idx[i]];
+        pl->ntiles = (int)sorted.size();
+        CK(dev_upload(pl, &pl->d_tiles, sorted.data(), sorted.size()));
+    }
+    // workspace
+    const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
+    const size_t per_pair = sizeof(float2) * (2 * spec_pp + ht_pp + z_pp);
+    int mp = params->max_pairs;
+    if (mp <= 0) {
+        const size_t budget = (size_t)2 << 30;
+        mp = (int)std::max<size_t>(1, std::min<size_t>(16, budget / per_pair));
+    }
+    pl->max_pairs = mp;
+    CK(dev_alloc(pl, &pl->d_spec1, spec_pp * mp));
+    CK(dev_alloc(pl, &pl->d_spec2, spec_pp * mp));
+    CK(dev_alloc(pl, &pl->d_ht, ht_pp * mp));
+    CK(dev_alloc(pl, &pl->d_z, z_pp * mp));
+#undef CK
+    *plan_out = pl;
+    return GR_OK;
+}
+
+int gr_plan_info(const gr_plan *pl, gr_info *info) {
+    if (!pl || !info) return fail(GR_EINVAL, "null argument");
+    const GrGeom &g = pl->g;
+    info->pdim = g.pdim;
+    info->M = g.M;
+    info->M0 = g.M0;
+    info->M02 = g.M02;
+    info->sampl = g.sampl;
+    info->L = g.L;
+    info->scale = g.scale;
+    info->n_det_eff = g.n_det_eff;
+    info->center_eff = g.center_eff;
+    info->out_off = g.out_off;
+    info->out_n = g.out_n;
+    info->max_pairs = pl->max_pairs;
+    info->device_bytes = pl->dev_bytes;
+    return GR_OK;
+}
+
+int gr_plan_tables(const gr_plan *pl, float *sine, float *cose, float *filphase, float *wtbl, float *winv) {
+    if (!pl) return fail(GR_EINVAL, "null plan");
+    GR_HIP(hipSetDevice(pl->prm.device));
+    const int n_ang = pl->prm.n_ang;
+    if (sine) GR_HIP(hipMemcpy(sine, pl->d_sine, sizeof(float) * n_ang, hipMemcpyDeviceToHost));
+    if (cose) GR_HIP(hipMemcpy(cose, pl->d_cose, sizeof(float) * n_ang, hipMemcpyDeviceToHost));
+    if (filphase) GR_HIP(hipMemcpy(filphase, pl->d_filphase, sizeof(float2) * pl->g.half, hipMemcpyDeviceToHost));
+    if (wtbl) GR_HIP(hipMemcpy(wtbl, pl->d_wtbl, sizeof(float) * (pl->prm.ltbl + 1), hipMemcpyDeviceToHost));
+    if (winv) GR_HIP(hipMemcpy(winv, pl->d_winv, sizeof(float) * pl->g.M0, hipMemcpyDeviceToHost));
+    return GR_OK;
+}
+
+int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride, float *out,
+                    long out_slice_stride, long out_row_stride, int n_slices, void *stream) {
+    if (!pl || !sino || !out) return fail(GR_EINVAL, "null argument");
+    if (n_slices < 0) return fail(GR_EINVAL, "negative slice count");
+    if (n_slices == 0) return GR_OK;
+    const GrGeom &g = pl->g;
+    if (sino_row_stride < g.n_det_in || sino_slice_stride < sino_row_stride * (long)pl->prm.n_ang)
+        return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
+    if (out_row_stride < g.out_n || out_slice_stride < out_row_stride * g.out_n)
+        return fail(GR_EINVAL, "image strides smaller than the window they hold");
+    hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
+    GR_HIP(hipSetDevice(pl->prm.device));
+    const GrDevTables t = pl->tables();
+    const int n_pairs = (n_slices + 1) / 2;
+    for (int p0 = 0; p0 < n_pairs; p0 += pl->max_pairs) {
+        const int np = std::min(pl->max_pairs, n_pairs - p0);
+        {
+            KernelTimer kt(pl, s, 0);
+            GR_HIP(gr_launch_rowfft(g, t, pl->prm.n_ang, sino, sino_slice_stride, sino_row_stride, n_slices, p0, np,
+                                    pl->d_spec1, pl->d_spec2, s));
+            kt.done();
+        }
+        {
+            KernelTimer kt(pl, s, 1);
+            GR_HIP(gr_launch_grid(g, t, pl->prm.n_ang, pl->prm.ltbl, pl->d_spec1, pl->d_spec2, pl->d_ht, np,
+                                  pl->prm.accum, s));
+            kt.done();
+        }
+        {
+            KernelTimer kt(pl, s, 2);
+            GR_HIP(gr_launch_ifft_rows(g, t, pl->d_ht, pl->d_z, np, s));
+            kt.done();
+        }
+        {
+            KernelTimer kt(pl, s, 3);
+            GR_HIP(gr_launch_ifft_cols(g, t, pl->d_z, out, out_slice_stride, out_row_stride, n_slices, p0, np, s));
+            kt.done();
+        }
+        if (pl->timing) pl->t.pairs += np;
+    }
+    return GR_OK;
+}
+
+int gr_recon_host(gr_plan *pl, const float *const *G1, const float *const *G2, float *const *S1,
+                  float *const *S2) {
+    if (!pl || !G1 || !G2 || !S1 || !S2) return fail(GR_EINVAL, "null argument");
+    const GrGeom &g = pl->g;
+    const int n_ang = pl->prm.n_ang, nd = g.n_det_in;
+    const long on = g.out_n;
+    const size_t in_n = (size_t)2 * n_ang * nd, out_n = (size_t)2 * on * on;
+    GR_HIP(hipSetDevice(pl->prm.device));
+    if (!pl->h_in) {
+        GR_HIP(hipHostMalloc((void **)&pl->h_in, sizeof(float) * in_n));
+        GR_HIP(hipHostMalloc((void **)&pl->h_out, sizeof(float) * out_n));
+        int rc = dev_alloc(pl, &pl->d_in, in_n);
+        if (rc) return rc;
+        rc = dev_alloc(pl, &pl->d_out, out_n);
+        if (rc) return rc;
+    }
+    for (int n = 0; n < n_ang; n++) {
+        memcpy(pl->h_in + (size_t)n * nd, G1[n], sizeof(float) * nd);
+        memcpy(pl->h_in + (size_t)(n_ang + n) * nd, G2[n], sizeof(float) * nd);
+    }
+    GR_HIP(hipMemcpyAsync(pl->d_in, pl->h_in, sizeof(float) * in_n, hipMemcpyHostToDevice, pl->stream));
+    int rc = gr_recon_device(pl, pl->d_in, (long)n_ang * nd, nd, pl->d_out, on * on, on, 2, pl->stream);
+    if (rc) return rc;
+    GR_HIP(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(float) * out_n, hipMemcpyDeviceToHost, pl->stream));
+    GR_HIP(hipStreamSynchronize(pl->stream));
+    for (long j = 0; j < on; j++) {
+        memcpy(S1[j], pl->h_out + j * on, sizeof(float) * on);
+        memcpy(S2[j], pl->h_out + (on + j) * on, sizeof(float) * on);
+    }
+    return GR_OK;
+}
+
+int gr_plan_destroy(gr_plan *pl) {
+    if (!pl) return GR_OK;
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    free_plan(pl);
+    return GR_OK;
+}
+
+int gr_set_timing(gr_plan *pl, int enable) {
+    if (!pl) return fail(GR_EINVAL, "null plan");
+    pl->timing = enable ? 1 : 0;
+    return GR_OK;
+}
+
+int gr_get_timing(gr_plan *pl, gr_timing *out, int reset) {
+    if (!pl || !out) return fail(GR_EINVAL, "null argument");
+    GR_HIP(hipSetDevice(pl->prm.device));
+    resolve_timing(pl);
+    *out = pl->t;
+    if (reset) pl->t = gr_timing{};
+    return GR_OK;
+}
+
+int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0, unsigned long long seed,
+                         int n_ang, int n_det, const float *centers, int n_ell, float photons, int density,
+                         void *stream) {
+    if (!sino || !centers || n_slices < 1 || n_ang < 1 || n_det < 1 || n_ell < 1 || n_ell > 64)
+        return fail(GR_EINVAL, "invalid phantom arguments (1 <= n_ell <= 64)");
+    if (slice_stride < (long)n_ang * n_det) return fail(GR_EINVAL, "slice stride too small");
+    hipStream_t s = (hipStream_t)stream;
+    float *dc = nullptr;
+    GR_HIP(hipMallocAsync((void **)&dc, sizeof(float) * n_slices, s));
+    GR_HIP(hipMemcpyAsync(dc, centers, sizeof(float) * n_slices, hipMemcpyHostToDevice, s));
+    hipError_t e = gr_launch_phantom(sino, slice_stride, n_slices, slice0, seed, n_ang, n_det, dc, n_ell, photons,
+                                     density, s);
+    hipError_t f = hipFreeAsync(dc, s);
+    hipError_t w = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "phantom kernels");
+    if (f != hipSuccess) return hip_fail(f, "hipFreeAsync");
+    if (w != hipSuccess) return hip_fail(w, "phantom sync");
+    return GR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// gr_fft.hpp -- power-of-two complex FFT of one row held in LDS.
+//
+// Stockham autosort formulation (N. K. Govindaraju et al., "High performance
+// discrete Fourier transforms on graphics processors", SC'08): passes of
+// radix 4, then one radix-2 pass when log2 N is odd; output in natural order.
+// Each pass reads all of a thread's butterfly inputs into registers, crosses a
+// barrier, and writes the outputs back into the same LDS row, so one buffer of
+// N complex values suffices.  SIGN = +1 computes sum_k x_k e^{+2 pi i jk/N},
+// SIGN = -1 the e^{-...} transform; neither is normalised (the conventions of
+// the reference's four1 with isign=+1 and fourn with isign=-1,
+// gridrec/GridrecCPP/Gridrec5.2/fftgr.cpp:8-120).
+//
+// tw[m] = e^{SIGN 2 pi i m / N}, m < N, rounded from double on the host.
+// MAXB bounds the radix-4 butterflies per thread: N/4 <= MAXB * nthreads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+__device__ __forceinline__ float2 gr_cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// 4-point DFT in place with root SIGN*i.
+template <int SIGN>
+__device__ __forceinline__ void gr_dft4(float2 &v0, float2 &v1, float2 &v2, float2 &v3) {
+    const float2 a0 = make_float2(v0.x + v2.x, v0.y + v2.y);
+    const float2 a1 = make_float2(v0.x - v2.x, v0.y - v2.y);
+    const float2 a2 = make_float2(v1.x + v3.x, v1.y + v3.y);
+    const float2 d = make_float2(v1.x - v3.x, v1.y - v3.y);
+    // a3 = d * (SIGN i)
+    const float2 a3 = SIGN > 0 ? make_float2(-d.y, d.x) : make_float2(d.y, -d.x);
+    v0 = make_float2(a0.x + a2.x, a0.y + a2.y);
+    v2 = make_float2(a0.x - a2.x, a0.y - a2.y);
+    v1 = make_float2(a1.x + a3.x, a1.y + a3.y);
+    v3 = make_float2(a1.x - a3.x, a1.y - a3.y);
+}
+
+template <int SIGN, int MAXB>
+__device__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr) {
+    int Ns = 1;
+    const int nb4 = N >> 2;
+    while (Ns * 4 <= N) {
+        float2 v[MAXB][4];
+        const int stride = N / (Ns * 4);
+#pragma unroll
+        for (int c = 0; c < MAXB; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb4) {
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[c][r] = buf[j + r * nb4];
+                if (Ns > 1) {
+                    const int m = k * stride;
+                    v[c][1] = gr_cmul(v[c][1], tw[m]);
+                    v[c][2] = gr_cmul(v[c][2], tw[2 * m]);
+                    v[c][3] = gr_cmul(v[c][3], tw[3 * m]);
+                }
+                gr_dft4<SIGN>(v[c][0], v[c][1], v[c][2], v[c][3]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < MAXB; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb4) {
+                const int k = j & (Ns - 1);
+                const int base = ((j - k) << 2) + k;
+#pragma unroll
+                for (int r = 0; r < 4; r++) buf[base + r * Ns] = v[c][r];
+            }
+        }
+        __syncthreads();
+        Ns <<= 2;
+    }
+    if (Ns < N) {  // N = 2 Ns: one radix-2 pass, twiddle e^{SIGN 2 pi i k/N}
+        const int nb2 = N >> 1;
+        float2 a[2 * MAXB], b[2 * MAXB];
+#pragma unroll
+        for (int c = 0; c < 2 * MAXB; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb2) {
+                const int k = j & (Ns - 1);
+                const float2 x0 = buf[j];
+                const float2 x1 = gr_cmul(buf[j + nb2], tw[k]);
+                a[c] = make_float2(x0.x + x1.x, x0.y + x1.y);
+                b[c] = make_float2(x0.x - x1.x, x0.y - x1.y);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 2 * MAXB; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb2) {
+                const int k = j & (Ns - 1);
+                const int base = ((j - k) << 1) + k;
+                buf[base] = a[c];
+                buf[base + Ns] = b[c];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Threads per FFT block and the matching MAXB for a row of N points.
+static inline int gr_fft_threads(long N) { return N >= 1024 ? 256 : (N >= 256 ? (int)(N / 4) : 64); }
+
+// Allow a kernel more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
+template <typename K>
+static inline hipError_t gr_allow_lds(K kernel, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}

@@ -1,0 +1,166 @@
+// gr_gather.hpp -- per-tap logic of the gather form of gridrec's gridding.
+//
+// The reference scatters every filtered sample (angle n, frequency j) into the
+// grid cells of an L x L box around (U, V) and its mirror
+// (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:276-302):
+//
+//   U = (scale*j)*COSE[n] + M2,  V = (scale*j)*SINE[n] + M2        (float)
+//   box: ceil(U-L2) <= iu <= floor(U+L2), ceil(V-L2) <= iv <= floor(V+L2),
+//        clipped to [1, M-1]
+//   w  = wtbl[(int)(|U-iu|*tblspcg + 0.5)] * wtbl[(int)(|V-iv|*tblspcg + 0.5)]
+//   H[iu][iv]     += w * Cdata1
+//   H[M-iu][M-iv] += w * Cdata2
+//
+// The gather evaluates, for one cell (a, b), exactly the same float
+// predicates and weights: the direct term uses (iu, iv) = (a, b) with Cdata1,
+// the mirrored term uses (iu, iv) = (M-a, M-b) with Cdata2 and the weight of
+// |U-(M-a)|, |V-(M-b)| -- the reference's weight for that deposit.  For an
+// integer a, ceil(x) <= a <=> x <= a and a <= floor(y) <=> a <= y, so the box
+// test is two float compares per axis; cells with a or b equal to 0 are never
+// written (the clip to [1, M-1]).
+//
+// All of it is __host__ __device__ so a CPU build can check it against the
+// reference's scatter; the file must be compiled without FP contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#pragma clang fp contract(off)
+
+struct GrGatherGeom {
+    float scale, L2, tblspcg, fM2;
+    int M, half, ltbl;
+};
+
+// (int)((double)X + 0.5) for X >= 0 without double arithmetic: X - trunc(X) is
+// exact in float, so rounding half up is a compare of that fraction with 0.5.
+__host__ __device__ __forceinline__ int gr_nn_index(float X) {
+    const int i = (int)X;
+    const float f = X - (float)i;
+    return i + (f >= 0.5f ? 1 : 0);
+}
+
+// If sample j of the angle with table values (C, S) deposits into cell
+// (qa, qb), return true and its weight.  qa, qb are exact small integers.
+__host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float qa, float qb,
+                                                const GrGatherGeom &g, const float *wtbl,
+                                                float &w) {
+    const float rt = g.scale * (float)j;
+    const float U = rt * C + g.fM2;
+    const float V = rt * S + g.fM2;
+    if (!(U - g.L2 <= qa && qa <= U + g.L2 && V - g.L2 <= qb && qb <= V + g.L2)) return false;
+    int iu = gr_nn_index(fabsf(U - qa) * g.tblspcg);
+    int iv = gr_nn_index(fabsf(V - qb) * g.tblspcg);
+    iu = iu > g.ltbl ? g.ltbl : iu;
+    iv = iv > g.ltbl ? g.ltbl : iv;
+    w = wtbl[iu] * wtbl[iv];
+    return true;
+}
+
+// Candidate samples of one angle for a cell at offset (du, dv) from the grid
+// centre: every j whose (U, V) can lie within L2 of the cell in both axes,
+// i.e. s = scale*j in {s : |s C - du| <= L2m, |s S - dv| <= L2m} with a margin
+// L2m > L2 covering float rounding of U, V.  The mirrored cell (M-a, M-b) sits
+// at (-du, -dv), whose interval is the negation of this one.  Returns the
+// direct range [jd0, jd1] and the mirrored range [jm0, jm1] (empty when
+// j0 > j1), both clipped to 1 .. half-1.
+__host__ __device__ __forceinline__ void gr_candidates(float du, float dv, float C, float S,
+                                                       float L2m, const GrGatherGeom &g,
+                                                       int &jd0, int &jd1, int &jm0, int &jm1) {
+    float lo = -1e9f, hi = 1e9f;
+    bool empty = false;
+    if (fabsf(C) > 1e-6f) {
+        const float ic = 1.0f / C;
+        const float x0 = (du - L2m) * ic, x1 = (du + L2m) * ic;
+        lo = fmaxf(lo, fminf(x0, x1));
+        hi = fminf(hi, fmaxf(x0, x1));
+    } else if (fabsf(du) > L2m) {
+        empty = true;
+    }
+    if (fabsf(S) > 1e-6f) {
+        const float is = 1.0f / S;
+        const float y0 = (dv - L2m) * is, y1 = (dv + L2m) * is;
+        lo = fmaxf(lo, fminf(y0, y1));
+        hi = fminf(hi, fmaxf(y0, y1));
+    } else if (fabsf(dv) > L2m) {
+        empty = true;
+    }
+    lo = fmaxf(lo, -1e9f);
+    hi = fminf(hi, 1e9f);
+    const float inv = 1.0f / g.scale;
+    const float jmax = (float)(g.half - 1);
+    if (empty || lo > hi || jmax < 1.0f) {
+        jd0 = jm0 = 1;
+        jd1 = jm1 = 0;
+        return;
+    }
+    // direct: s in [lo, hi]; mirrored: s in [-hi, -lo]; s > 0 only
+    const float d0 = fmaxf(1.0f, ceilf(lo * inv - 1e-3f));
+    const float d1 = fminf(jmax, floorf(hi * inv + 1e-3f));
+    const float m0 = fmaxf(1.0f, ceilf(-hi * inv - 1e-3f));
+    const float m1 = fminf(jmax, floorf(-lo * inv + 1e-3f));
+    if (d0 <= d1) { jd0 = (int)d0; jd1 = (int)d1; } else { jd0 = 1; jd1 = 0; }
+    if (m0 <= m1) { jm0 = (int)m0; jm1 = (int)m1; } else { jm0 = 1; jm1 = 0; }
+}
+
+// Angle window of an 8 x 8 tile with origin (tx*8, ty*8): the ranges
+// [r0, r1) and [r2, r3) of sorted angle keys (akey, ascending in [0, pi)) whose
+// sample line passes within dmax of the tile centre.  A line of direction
+// theta through the grid centre is at distance R |sin(phi - theta)| from a
+// point at polar (R, phi), so the window is |phi - theta| <= asin(dmax/R)
+// modulo pi; central tiles take every angle.
+__host__ __device__ __forceinline__ int gr_lower_bound(const float *a, int n, float x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__host__ __device__ __forceinline__ int gr_upper_bound(const float *a, int n, float x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__host__ __device__ __forceinline__ void gr_angle_window(int tx, int ty, float M2, float dmax,
+                                                         const float *akey, int n_ang, int &r0,
+                                                         int &r1, int &r2, int &r3) {
+    const float cu = (float)(tx * 8) + 3.5f - M2, cv = (float)(ty * 8) + 3.5f - M2;
+    const float Rt = sqrtf(cu * cu + cv * cv);
+    r0 = 0;
+    r1 = n_ang;
+    r2 = 0;
+    r3 = 0;
+    if (Rt <= dmax + 1.0f) return;
+    const float kPi = 3.14159265f;
+    float phi = atan2f(cv, cu);
+    if (phi < 0.0f) phi += kPi;
+    if (phi >= kPi) phi -= kPi;
+    const float alpha = asinf(fminf(1.0f, dmax / Rt)) + 1e-4f;
+    const float lo = phi - alpha, hi = phi + alpha;
+    if (lo < 0.0f) {
+        r0 = gr_lower_bound(akey, n_ang, lo + kPi);
+        r3 = gr_upper_bound(akey, n_ang, hi);
+    } else if (hi >= kPi) {
+        r0 = gr_lower_bound(akey, n_ang, lo);
+        r3 = gr_upper_bound(akey, n_ang, hi - kPi);
+    } else {
+        r0 = gr_lower_bound(akey, n_ang, lo);
+        r1 = gr_upper_bound(akey, n_ang, hi);
+    }
+}
+
+// Error-free transformation (Knuth's TwoSum): s + x = t + e exactly.
+__host__ __device__ __forceinline__ void gr_two_sum(float &s, float &c, float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    const float e = (s - (t - bp)) + (x - bp);
+    s = t;
+    c += e;
+}

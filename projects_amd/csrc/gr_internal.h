@@ -1,0 +1,84 @@
+// gr_internal.h -- types shared by the host C-ABI and the HIP kernels of the
+// MI355X gridrec library.  Not installed; the public interface is
+// include/gridrec_hip.h (C-ABI) and include/grid.h (reference C++ API).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/gridrec_hip.h"
+
+// Sizes and scalars of one geometry, as recon_init derives them
+// (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:87-120, 223-226).
+struct GrGeom {
+    long pdim, M, M0, M02, half;  // half = pdim/2; samples j = 1 .. half-1 are gridded
+    float sampl;                  // adjusted M/M0
+    float L, scale, L2, tblspcg;
+    int flag;                     // ROI offset present
+    int n_det_in;                 // width of the caller's rows
+    int n_det_eff;                // width after padding (what grid.cpp calls n_det)
+    float center_eff;             // centre after padding
+    int pad;                      // GR_PAD_*
+    int filter_id;
+    gr_pswf pswf;
+    long out_off, out_n;          // output window inside M0 x M0
+};
+
+// Host side (gr_tables.cpp): sizes and the reference's float tables.
+int gr_geom_from_params(const gr_params *p, GrGeom *g, const char **why);
+int gr_find_pswf(float C, gr_pswf *out);
+int gr_filter_id_of(const char *name);
+// sine/cose: n_ang; filphase: 2*half floats; wtbl: ltbl+1; winv: M0;
+// roi_offset: n_ang (phase offset per angle, grid.cpp:237; zeros without ROI).
+void gr_host_tables(const gr_params *p, const GrGeom &g, float *sine, float *cose,
+                    float *filphase, float *wtbl, float *winv, float *roi_offset);
+
+// Device tables and workspace of a plan.
+struct GrDevTables {
+    const float *cose, *sine;      // n_ang
+    const float *roi_offset;       // n_ang
+    const float2 *filphase;        // half
+    const float *wtbl;             // ltbl+1
+    const float *winv;             // M0
+    const float2 *tw_row;          // pdim: e^{+2 pi i m/pdim}
+    const float2 *tw_grid;         // M:    e^{-2 pi i m/M}
+    const float *akey;             // n_ang angle keys in [0, pi), sorted
+    const int *aord;               // n_ang angle index of each sorted key
+    const unsigned *tiles;         // tile list (tx | ty << 16), nearest-to-centre first
+    int ntiles;
+};
+
+// ---- kernel launchers (one per .hip file) ----------------------------------
+
+// K1: row FFT (+1), filter/phase, Hermitian split.  spec1/spec2 are
+// [pairs][n_ang][half] complex; pair p reads slices 2*(pair0+p) and +1.
+hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, const float *sino,
+                            long slice_stride, long row_stride, int n_slices, int pair0, int pairs,
+                            float2 *spec1, float2 *spec2, hipStream_t s);
+
+// K2: gather gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
+hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
+                          const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
+                          int accum, hipStream_t s);
+
+// K3: FFT (-1) of each row of Ht (over a), output rows of the window only:
+// Z[pairs][M][out_n].
+hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
+                               int pairs, hipStream_t s);
+
+// K4: FFT (-1) down the columns of Z (over b), deapodization and crop into the
+// caller's images.  Pair p writes slices 2*(pair0+p) (real part) and +1 (imag)
+// unless that slice index is >= n_slices.
+hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
+                               long out_slice_stride, long out_row_stride, int n_slices, int pair0,
+                               int pairs, hipStream_t s);
+
+// FP64 tables on the device, rounded to float: filphase (half), wtbl (ltbl+1),
+// winv (M0).
+hipError_t gr_launch_tables_fp64(const GrGeom &g, const gr_params *p, float2 *filphase, float *wtbl,
+                                 float *winv, hipStream_t s);
+
+// Synthetic sinograms.
+struct GrEllipse { float a, b, rho, phi, x0, y0; };
+hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long slice0,
+                             unsigned long long seed, int n_ang, int n_det, const float *centers_dev,
+                             int n_ell, float photons, int density, hipStream_t s);

@@ -1,0 +1,147 @@
+// k_ifft2.hip -- K3 + K4: 2D inverse FFT of the grid, deapodization and crop.
+//
+// Replaces phases 2 and 3 of do_recon (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:311-411):
+//   fourn(H, {M, M}, 2, -1)                      x[p][q] = sum_{u,v} H[u][v] e^{-2 pi i (up+vq)/M}
+//   S1[j][k] = winv[j]*winv[k] * Re x[iu(j)][iv(k)]
+//   S2[j][k] = winv[j]*winv[k] * Im x[iu(j)][iv(k)]
+// where iu(j) = M-M02+j for j < M02 and j-M02 after (the four-corner wrap).
+//
+// Only the requested window [off, off+n) of rows and columns is produced, so
+// both passes are pruned: K3 transforms every row of Ht (one per b, over a)
+// but keeps only the n outputs p = iu(j) of the window; K4 transforms those
+// n columns down b, keeps the n outputs q = iv(k) and writes the two images
+// row by row with the deapodization product fused in.
+#include "gr_fft.hpp"
+#include "gr_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ __forceinline__ long corner_index(long j, long M, long M02) {
+    return j < M02 ? M - M02 + j : j - M02;
+}
+
+struct RowsArgs {
+    const float2 *ht;
+    float2 *z;
+    const float2 *tw;
+    int M;
+    long M02, out_off, out_n;
+};
+
+template <int MAXB>
+__global__ void __launch_bounds__(256) k_ifft_rows(RowsArgs A) {
+    extern __shared__ float2 row[];
+    const int b = blockIdx.x, p = blockIdx.y;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const float2 *src = A.ht + ((long)p * A.M + b) * A.M;
+    for (int i = tid; i < A.M; i += nthr) row[i] = src[i];
+    __syncthreads();
+    gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr);
+    float2 *dst = A.z + ((long)p * A.M + b) * A.out_n;
+    for (long jj = tid; jj < A.out_n; jj += nthr) dst[jj] = row[corner_index(A.out_off + jj, A.M, A.M02)];
+}
+
+struct ColsArgs {
+    const float2 *z;
+    float *out;
+    const float2 *tw;
+    const float *winv;
+    long out_slice_stride, out_row_stride;
+    int M, cw, n_slices, pair0;
+    long M02, out_off, out_n;
+};
+
+template <int MAXB>
+__global__ void __launch_bounds__(256) k_ifft_cols(ColsArgs A) {
+    extern __shared__ float2 cols[];  // cw columns of M
+    const int p = blockIdx.y;
+    const long c0 = (long)blockIdx.x * A.cw;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int M = A.M, cw = A.cw;
+    const float2 *zp = A.z + (long)p * M * A.out_n;
+    for (long idx = tid; idx < (long)cw * M; idx += nthr) {
+        const long b = idx / cw, c = idx - b * cw;
+        cols[c * M + b] = (c0 + c < A.out_n) ? zp[b * A.out_n + c0 + c] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + (long)c * M, M, A.tw, tid, nthr);
+
+    const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
+    float *o1 = A.out + s1 * A.out_slice_stride;
+    float *o2 = s2 < A.n_slices ? A.out + s2 * A.out_slice_stride : nullptr;
+    for (int c = 0; c < cw; c++) {
+        const long jj = c0 + c;
+        if (jj >= A.out_n) break;
+        const float cu = A.winv[A.out_off + jj];
+        const float2 *col = cols + (long)c * M;
+        for (long kk = tid; kk < A.out_n; kk += nthr) {
+            const long k = A.out_off + kk;
+            const float corrn = cu * A.winv[k];  // grid.cpp:401
+            const float2 v = col[corner_index(k, M, A.M02)];
+            o1[jj * A.out_row_stride + kk] = corrn * v.x;
+            if (o2) o2[jj * A.out_row_stride + kk] = corrn * v.y;
+        }
+    }
+}
+
+inline int maxb_of(long N, int nthr) { return (int)((N / 4 + nthr - 1) / nthr); }
+
+}  // namespace
+
+hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
+                               int pairs, hipStream_t s) {
+    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.M02, g.out_off, g.out_n};
+    const int nthr = gr_fft_threads(g.M);
+    const dim3 grid((unsigned)g.M, pairs), block(nthr);
+    const size_t lds = sizeof(float2) * g.M;
+    switch (maxb_of(g.M, nthr)) {
+    case 0:
+    case 1: { hipError_t e_ = gr_allow_lds(k_ifft_rows<1>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<1>, grid, block, lds, s, A); } break;
+    case 2: { hipError_t e_ = gr_allow_lds(k_ifft_rows<2>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<2>, grid, block, lds, s, A); } break;
+    case 4: { hipError_t e_ = gr_allow_lds(k_ifft_rows<4>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<4>, grid, block, lds, s, A); } break;
+    case 8: { hipError_t e_ = gr_allow_lds(k_ifft_rows<8>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<8>, grid, block, lds, s, A); } break;
+    case 16: { hipError_t e_ = gr_allow_lds(k_ifft_rows<16>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<16>, grid, block, lds, s, A); } break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Columns per workgroup: as many M-point columns as fit in 128 KiB of LDS.
+static int cols_per_block(long M) {
+    long cw = 16384 / M;
+    return (int)(cw < 1 ? 1 : (cw > 16 ? 16 : cw));
+}
+
+hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
+                               long out_slice_stride, long out_row_stride, int n_slices, int pair0,
+                               int pairs, hipStream_t s) {
+    ColsArgs A;
+    A.z = z;
+    A.out = out;
+    A.tw = t.tw_grid;
+    A.winv = t.winv;
+    A.out_slice_stride = out_slice_stride;
+    A.out_row_stride = out_row_stride;
+    A.M = (int)g.M;
+    A.cw = cols_per_block(g.M);
+    A.n_slices = n_slices;
+    A.pair0 = pair0;
+    A.M02 = g.M02;
+    A.out_off = g.out_off;
+    A.out_n = g.out_n;
+    const int nthr = gr_fft_threads(g.M);
+    const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(nthr);
+    const size_t lds = sizeof(float2) * A.cw * g.M;
+    switch (maxb_of(g.M, nthr)) {
+    case 0:
+    case 1: { hipError_t e_ = gr_allow_lds(k_ifft_cols<1>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<1>, grid, block, lds, s, A); } break;
+    case 2: { hipError_t e_ = gr_allow_lds(k_ifft_cols<2>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<2>, grid, block, lds, s, A); } break;
+    case 4: { hipError_t e_ = gr_allow_lds(k_ifft_cols<4>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<4>, grid, block, lds, s, A); } break;
+    case 8: { hipError_t e_ = gr_allow_lds(k_ifft_cols<8>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<8>, grid, block, lds, s, A); } break;
+    case 16: { hipError_t e_ = gr_allow_lds(k_ifft_cols<16>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<16>, grid, block, lds, s, A); } break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

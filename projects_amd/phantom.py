@@ -130,6 +130,24 @@ def _ptrs_constants(lam):
     return a, b, inv_alpha, vr
 
 
+def log_poisson_pmf(k: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """k log(lam) - lam - log(k!), through the Stirling series of log Gamma(k+1)
+    for k >= 9 (the form the GPU generator evaluates in float)."""
+    from math import lgamma
+
+    k = np.asarray(k, np.float64)
+    lam = np.asarray(lam, np.float64)
+    z = k + 1.0
+    big = k >= 9
+    out = np.empty(np.broadcast(k, lam).shape)
+    zb, lb, kb = np.broadcast_to(z, out.shape)[big], np.broadcast_to(lam, out.shape)[big], np.broadcast_to(k, out.shape)[big]
+    out[big] = (kb * np.log1p((lb - zb) / zb) + (zb - lb) - 0.5 * np.log(2 * math.pi * zb) - 1 / (12 * zb)
+                + 1 / (360 * zb ** 3))
+    ks, ls = np.broadcast_to(k, out.shape)[~big], np.broadcast_to(lam, out.shape)[~big]
+    out[~big] = ks * np.log(ls) - ls - np.array([lgamma(x + 1.0) if x >= 0 else 0.0 for x in ks.ravel()])
+    return out
+
+
 def poisson(lam: np.ndarray, seed: int, slice_idx: int, max_rounds: int = 64) -> np.ndarray:
     """Poisson(lam) draws for an (n_ang, n_det) array of means.
 
@@ -140,14 +158,11 @@ def poisson(lam: np.ndarray, seed: int, slice_idx: int, max_rounds: int = 64) ->
     Round r of detector k, angle n uses counter (k, n, slice, 2 + r//2);
     the even round takes words 0-1, the odd round words 2-3.
     """
-    from math import lgamma
-
     lam = np.asarray(lam, np.float64)
     n_ang, n_det = lam.shape
     k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
     kk, nn = np.meshgrid(np.arange(n_det, dtype=np.uint32), np.arange(n_ang, dtype=np.uint32))
     out = np.zeros(lam.shape, np.float64)
-    done = np.zeros(lam.shape, bool)
 
     small = lam < 10.0
     if small.any():
@@ -166,13 +181,11 @@ def poisson(lam: np.ndarray, seed: int, slice_idx: int, max_rounds: int = 64) ->
             active = active & (u > F)
             it += 1
         out[small] = k
-        done[small] = True
 
     big = ~small
     if big.any():
         lb = lam[big]
         a, b, inv_alpha, vr = _ptrs_constants(lb)
-        loglam = np.log(lb)
         res = np.full(lb.shape, -1.0)
         kb, nb = kk[big], nn[big]
         for r in range(max_rounds):
@@ -184,13 +197,12 @@ def poisson(lam: np.ndarray, seed: int, slice_idx: int, max_rounds: int = 64) ->
                 U, V = u01(w[0]) - 0.5, u01(w[1])
             else:
                 U, V = u01(w[2]) - 0.5, u01(w[3])
-            la, aa, ba, ia, va, lla = lb[act], a[act], b[act], inv_alpha[act], vr[act], loglam[act]
+            la, aa, ba, ia, va = lb[act], a[act], b[act], inv_alpha[act], vr[act]
             us = 0.5 - np.abs(U)
             k = np.floor((2.0 * aa / us + ba) * U + la + 0.43)
             acc = (us >= 0.07) & (V <= va)
             rej = (k < 0) | ((us < 0.013) & (V > us))
-            lg = np.array([lgamma(x + 1.0) if x >= 0 else 0.0 for x in k.ravel()]).reshape(k.shape)
-            test = (np.log(V) + np.log(ia) - np.log(aa / (us * us) + ba)) <= (-la + k * lla - lg)
+            test = (np.log(V) + np.log(ia) - np.log(aa / (us * us) + ba)) <= log_poisson_pmf(np.maximum(k, 0), la)
             ok = acc | (~rej & test)
             cur = res[act]
             cur[ok] = k[ok]
