@@ -167,7 +167,7 @@ def main():
     # CPU baseline: the oracle's FP32 restatement on this host's cores, bounded sample
     if not a.no_cpu:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        pairs = a.cpu_pairs or threads
+        pairs = a.cpu_pairs or 8 * threads          # ~10-20 s of CPU work at this size
         pairs = min(pairs, ns // 2)
         s = sino[0:2 * pairs].cpu().numpy()
         g = O.Geometry(n_ang=nang, n_det=ndet, center=center)
