@@ -43,7 +43,22 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 4 plain device copies of 1 GiB (reference for the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy of known size: 1 GiB read + 1 GiB written per launch, the
+    scale for FETCH_SIZE / WRITE_SIZE counter readings."""
+    import torch
+
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_per_launch": 2 * src.numel() * 4, "launches": 4}), flush=True)
 
 
 def kernel_bytes(kind, n_ang, n_det, half, M, out_n, pairs):
@@ -63,6 +78,9 @@ def kernel_bytes(kind, n_ang, n_det, half, M, out_n, pairs):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(slice pairs are independent: several GPUs would each take a share of the batch)", file=sys.stderr)

@@ -51,6 +51,7 @@ struct gr_plan {
     float2 *d_filphase = nullptr, *d_tw_row = nullptr, *d_tw_grid = nullptr;
     float *d_akey = nullptr;
     int *d_aord = nullptr;
+    float4 *d_ang = nullptr;
     unsigned *d_tiles = nullptr;
     int ntiles = 0;
     // workspace
@@ -81,6 +82,7 @@ struct gr_plan {
         t.tw_grid = d_tw_grid;
         t.akey = d_akey;
         t.aord = d_aord;
+        t.ang = d_ang;
         t.tiles = d_tiles;
         t.ntiles = ntiles;
         return t;
@@ -108,7 +110,7 @@ int dev_alloc(gr_plan *pl, T **dst, size_t n) {
 void free_plan(gr_plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase,
-                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_tiles, pl->d_spec1,
+                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_ang, pl->d_tiles, pl->d_spec1,
                     pl->d_spec2, pl->d_ht, pl->d_z, pl->d_in, pl->d_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -287,6 +289,12 @@ int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
         for (int i = 1; i < n_ang; i++) akey[i] = std::max(akey[i], akey[i - 1]);
         CK(dev_upload(pl, &pl->d_akey, akey.data(), n_ang));
         CK(dev_upload(pl, &pl->d_aord, ord.data(), n_ang));
+        std::vector<float4> ang(n_ang);
+        for (int i = 0; i < n_ang; i++) {
+            const float c = cose[ord[i]], sn = sine[ord[i]];
+            ang[i] = make_float4(c, sn, fabsf(c) > 1e-6f ? 1.0f / c : 0.0f, fabsf(sn) > 1e-6f ? 1.0f / sn : 0.0f);
+        }
+        CK(dev_upload(pl, &pl->d_ang, ang.data(), n_ang));
     }
     // 8x8 tiles of the M x M grid, nearest to the centre first
     {

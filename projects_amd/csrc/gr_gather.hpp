@@ -156,6 +156,52 @@ __host__ __device__ __forceinline__ void gr_angle_window(int tx, int ty, float M
     }
 }
 
+// gr_candidates with precomputed reciprocals: ic = 1/C and is = 1/S, or 0 when
+// |C| (|S|) <= 1e-6, where that axis gives no bound on s (the samples stay
+// within 2048*1e-6 of the centre line) and the cell must lie within L2m of it.
+__host__ __device__ __forceinline__ void gr_candidates_r(float du, float dv, float ic, float is,
+                                                         float L2m, const GrGatherGeom &g,
+                                                         int &jd0, int &jd1, int &jm0, int &jm1) {
+    float lo = -1e9f, hi = 1e9f;
+    bool empty = false;
+    if (ic != 0.0f) {
+        const float x0 = (du - L2m) * ic, x1 = (du + L2m) * ic;
+        lo = fmaxf(lo, fminf(x0, x1));
+        hi = fminf(hi, fmaxf(x0, x1));
+    } else if (fabsf(du) > L2m) {
+        empty = true;
+    }
+    if (is != 0.0f) {
+        const float y0 = (dv - L2m) * is, y1 = (dv + L2m) * is;
+        lo = fmaxf(lo, fminf(y0, y1));
+        hi = fminf(hi, fmaxf(y0, y1));
+    } else if (fabsf(dv) > L2m) {
+        empty = true;
+    }
+    const float inv = 1.0f / g.scale;
+    const float jmax = (float)(g.half - 1);
+    jd0 = jm0 = 1;
+    jd1 = jm1 = 0;
+    if (empty || lo > hi || jmax < 1.0f) return;
+    lo = fmaxf(lo, -1e9f);
+    hi = fminf(hi, 1e9f);
+    const float d0 = fmaxf(1.0f, ceilf(lo * inv - 1e-3f));
+    const float d1 = fminf(jmax, floorf(hi * inv + 1e-3f));
+    const float m0 = fmaxf(1.0f, ceilf(-hi * inv - 1e-3f));
+    const float m1 = fminf(jmax, floorf(-lo * inv + 1e-3f));
+    if (d0 <= d1) { jd0 = (int)d0; jd1 = (int)d1; }
+    if (m0 <= m1) { jm0 = (int)m0; jm1 = (int)m1; }
+}
+
+// Kahan compensated summation: c carries the rounding error of the running
+// sum s with opposite sign; the total is s - c.
+__host__ __device__ __forceinline__ void gr_kahan(float &s, float &c, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
 // Error-free transformation (Knuth's TwoSum): s + x = t + e exactly.
 __host__ __device__ __forceinline__ void gr_two_sum(float &s, float &c, float x) {
     const float t = s + x;

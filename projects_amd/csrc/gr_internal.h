@@ -43,6 +43,7 @@ struct GrDevTables {
     const float2 *tw_grid;         // M:    e^{-2 pi i m/M}
     const float *akey;             // n_ang angle keys in [0, pi), sorted
     const int *aord;               // n_ang angle index of each sorted key
+    const float4 *ang;             // n_ang in sorted order: COSE, SINE, 1/COSE, 1/SINE (0 if |.| <= 1e-6)
     const unsigned *tiles;         // tile list (tx | ty << 16), nearest-to-centre first
     int ntiles;
 };

@@ -12,15 +12,23 @@
 // within dmax = 3.5*sqrt2 + (L/2)*sqrt2 (+ margin) of the tile centre.  For a
 // tile at radius R and polar angle phi that is the set of line angles within
 // asin(dmax/R) of phi modulo pi; angle keys atan2(SINE, COSE) mod pi are sorted
-// on the host and the window is found by binary search, wave-uniform.
-// Central tiles (R <= dmax + 1) scan every angle.  Tiles are listed nearest to
-// the centre first, so the heaviest work is dispatched first.
+// on the host and the window is found by binary search, wave-uniform.  Each
+// angle of the window is then tested exactly (perpendicular distance against
+// the tile's half-width in the line's normal direction) and skipped when its
+// line misses the tile.  Central tiles scan every angle; tiles beyond the
+// largest sample radius are written as zeros.  Tiles are listed nearest to the
+// centre first, so the heaviest work is dispatched first.
 //
-// Accumulation: GR_ACCUM_COMPENSATED keeps a TwoSum error term per component
-// (the sum of float products is then exact to within one rounding of the
-// final add); GR_ACCUM_PLAIN adds in float like the reference.
+// Per angle, the samples that can reach the tile form one contiguous range of
+// j for the direct term and one for the mirrored term; the wave stages those
+// spectra in its own LDS slot (64 at a time) and each lane then runs its own
+// exact candidate range against LDS, so no tap waits on a global load.
+//
+// Accumulation: GR_ACCUM_COMPENSATED keeps a Kahan correction per component;
+// GR_ACCUM_PLAIN adds in float like the reference.
 //
 // Output: Ht[pair][b][a] = H[a][b] (transposed, so K3's rows run over a).
+#include "gr_fft.hpp"
 #include "gr_gather.hpp"
 #include "gr_internal.h"
 
@@ -31,72 +39,110 @@ namespace {
 struct GridArgs {
     const float2 *spec1, *spec2;
     float2 *ht;
-    const float *cose, *sine, *akey;
-    const int *aord;
+    const float4 *ang;   // sorted: (COSE, SINE, 1/COSE or 0, 1/SINE or 0)
+    const int *aord;     // sorted -> angle index
+    const float *akey;   // sorted keys
     const unsigned *tiles;
     const float *wtbl;
-    int ntiles, n_ang, half;
+    int ntiles, n_ang, half, lds_wtbl;
     GrGatherGeom g;
-    float L2m, dmax;
+    float L2m, dmax, rmax;
 };
 
 template <int ACCUM>
+__device__ __forceinline__ void accumulate(float &sr, float &cr, float &si, float &ci, float xr, float xi) {
+    if (ACCUM == GR_ACCUM_COMPENSATED) {
+        gr_kahan(sr, cr, xr);
+        gr_kahan(si, ci, xi);
+    } else {
+        sr += xr;
+        si += xi;
+    }
+}
+
+// Taps of samples [j0, j1] of one angle into cell (qa, qb); spectra of samples
+// c0 .. c0+63 are staged in stage[].
+template <int ACCUM>
+__device__ __forceinline__ void run_taps(int j0, int j1, int c0, float C, float S, float qa, float qb,
+                                         const GrGatherGeom &g, const float *wtbl, const float2 *stage,
+                                         float &sr, float &cr, float &si, float &ci) {
+    for (int j = j0; j <= j1; j++) {
+        float w;
+        const float2 v = stage[j - c0];
+        if (gr_tap(j, C, S, qa, qb, g, wtbl, w)) accumulate<ACCUM>(sr, cr, si, ci, w * v.x, w * v.y);
+    }
+}
+
+template <int ACCUM>
 __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
-    extern __shared__ float s_wtbl[];
+    extern __shared__ float smem[];
+    float *s_wtbl = smem;
     for (int i = threadIdx.x; i <= A.g.ltbl; i += blockDim.x) s_wtbl[i] = A.wtbl[i];
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ti = blockIdx.x * 4 + wave;
     if (ti >= A.ntiles) return;
+    float2 *stage = reinterpret_cast<float2 *>(smem + A.lds_wtbl) + wave * 64;
     const int p = blockIdx.y;
     const unsigned tt = A.tiles[ti];
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * 8 + (lane & 7), b = ty * 8 + (lane >> 3);
     const int M = A.g.M;
     const float M2 = A.g.fM2;
-
-    // angle window of the tile (wave-uniform)
-    int r0, r1, r2, r3;
-    gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
+    const float cu = (float)(tx * 8) + 3.5f - M2, cv = (float)(ty * 8) + 3.5f - M2;
 
     const bool valid = a >= 1 && b >= 1 && a < M && b < M;
     const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
     const float du = pa - M2, dv = pb - M2;
+    const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
     float sr = 0.0f, cr = 0.0f, si = 0.0f, ci = 0.0f;
-    const float2 *sp1 = A.spec1 + (long)p * A.n_ang * A.half;
-    const float2 *sp2 = A.spec2 + (long)p * A.n_ang * A.half;
 
-    for (int part = 0; part < 2; part++) {
-        const int k0 = part ? r2 : r0, k1 = part ? r3 : r1;
-        for (int k = k0; k < k1; k++) {
-            const int n = A.aord[k];
-            const float C = A.cose[n], S = A.sine[n];
-            int jd0, jd1, jm0, jm1;
-            gr_candidates(du, dv, C, S, A.L2m, A.g, jd0, jd1, jm0, jm1);
-            const int cd = valid ? max(0, jd1 - jd0 + 1) : 0;
-            const int cm = valid ? max(0, jm1 - jm0 + 1) : 0;
-            const float2 *row1 = sp1 + (long)n * A.half;
-            const float2 *row2 = sp2 + (long)n * A.half;
-            for (int i = 0; i < cd + cm; i++) {
-                const bool mir = i >= cd;
-                const int j = mir ? jm0 + (i - cd) : jd0 + i;
-                float w;
-                if (gr_tap(j, C, S, mir ? ma : pa, mir ? mb : pb, A.g, s_wtbl, w)) {
-                    const float2 v = mir ? row2[j] : row1[j];
-                    const float xr = w * v.x, xi = w * v.y;
-                    if (ACCUM == GR_ACCUM_COMPENSATED) {
-                        gr_two_sum(sr, cr, xr);
-                        gr_two_sum(si, ci, xi);
-                    } else {
-                        sr += xr;
-                        si += xi;
-                    }
+    if (sqrtf(cu * cu + cv * cv) <= A.rmax) {
+        int r0, r1, r2, r3;
+        gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
+        const float2 *base1 = A.spec1 + (long)p * A.n_ang * A.half;
+        const float2 *base2 = A.spec2 + (long)p * A.n_ang * A.half;
+        for (int part = 0; part < 2; part++) {
+            const int k0 = part ? r2 : r0, k1 = part ? r3 : r1;
+            for (int k = k0; k < k1; k++) {
+                const float4 cs = A.ang[k];
+                // does the line pass through the tile grown by the box half-width?
+                if (fabsf(cu * cs.y - cv * cs.x) > reach * (fabsf(cs.x) + fabsf(cs.y)) + 0.05f) continue;
+                int td0, td1, tm0, tm1;
+                gr_candidates_r(cu, cv, cs.z, cs.w, reach, A.g, td0, td1, tm0, tm1);
+                if (td0 > td1 && tm0 > tm1) continue;
+                int jd0, jd1, jm0, jm1;
+                gr_candidates_r(du, dv, cs.z, cs.w, A.L2m, A.g, jd0, jd1, jm0, jm1);
+                if (!valid) {
+                    jd1 = jd0 - 1;
+                    jm1 = jm0 - 1;
+                }
+                const int n = A.aord[k];
+                const float2 *row1 = base1 + (long)n * A.half;
+                const float2 *row2 = base2 + (long)n * A.half;
+                for (int c0 = td0; c0 <= td1; c0 += 64) {
+                    const int jj = c0 + lane;
+                    stage[lane] = jj <= td1 ? row1[jj] : make_float2(0.0f, 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    run_taps<ACCUM>(max(jd0, c0), min(min(jd1, td1), c0 + 63), c0, cs.x, cs.y, pa, pb, A.g, s_wtbl, stage,
+                                    sr, cr, si, ci);
+                    __builtin_amdgcn_wave_barrier();
+                }
+                for (int c0 = tm0; c0 <= tm1; c0 += 64) {
+                    const int jj = c0 + lane;
+                    stage[lane] = jj <= tm1 ? row2[jj] : make_float2(0.0f, 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    run_taps<ACCUM>(max(jm0, c0), min(min(jm1, tm1), c0 + 63), c0, cs.x, cs.y, ma, mb, A.g, s_wtbl, stage,
+                                    sr, cr, si, ci);
+                    __builtin_amdgcn_wave_barrier();
                 }
             }
         }
     }
-    if (a < M && b < M) A.ht[((long)p * M + b) * M + a] = make_float2(sr + cr, si + ci);
+    const float hr = ACCUM == GR_ACCUM_COMPENSATED ? sr - cr : sr;
+    const float hi = ACCUM == GR_ACCUM_COMPENSATED ? si - ci : si;
+    if (a < M && b < M) A.ht[((long)p * M + b) * M + a] = make_float2(hr, hi);
 }
 
 }  // namespace
@@ -108,15 +154,15 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     A.spec1 = spec1;
     A.spec2 = spec2;
     A.ht = ht;
-    A.cose = t.cose;
-    A.sine = t.sine;
-    A.akey = t.akey;
+    A.ang = t.ang;
     A.aord = t.aord;
+    A.akey = t.akey;
     A.tiles = t.tiles;
     A.wtbl = t.wtbl;
     A.ntiles = t.ntiles;
     A.n_ang = n_ang;
     A.half = (int)g.half;
+    A.lds_wtbl = (int)((ltbl + 1 + 1) & ~1L);  // keep the staging area 8-byte aligned
     A.g.scale = g.scale;
     A.g.L2 = g.L2;
     A.g.tblspcg = g.tblspcg;
@@ -126,11 +172,17 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     A.g.ltbl = (int)ltbl;
     A.L2m = g.L2 + 0.01f;
     A.dmax = 3.5f * 1.41421356f + A.L2m * 1.41421356f + 0.05f;
+    // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
+    A.rmax = g.scale * (float)(g.half - 1) * 1.0001f + A.dmax + 0.5f;
     const dim3 grid((t.ntiles + 3) / 4, pairs), block(256);
-    const size_t lds = sizeof(float) * (ltbl + 1);
-    if (accum == GR_ACCUM_PLAIN)
+    const size_t lds = sizeof(float) * A.lds_wtbl + sizeof(float2) * 4 * 64;
+    hipError_t e;
+    if (accum == GR_ACCUM_PLAIN) {
+        if ((e = gr_allow_lds(k_grid<GR_ACCUM_PLAIN>, lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_grid<GR_ACCUM_PLAIN>, grid, block, lds, s, A);
-    else
+    } else {
+        if ((e = gr_allow_lds(k_grid<GR_ACCUM_COMPENSATED>, lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_grid<GR_ACCUM_COMPENSATED>, grid, block, lds, s, A);
+    }
     return hipGetLastError();
 }

@@ -76,6 +76,16 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
     for (int i = 0; i < n_ang; i++) akey[i] = (float)key[ord[i]];
     for (int i = 1; i < n_ang; i++) akey[i] = std::max(akey[i], akey[i - 1]);
 
+    // per-angle table in sorted order, as the plan uploads it
+    std::vector<float> C4(4 * (size_t)n_ang);
+    for (int i = 0; i < n_ang; i++) {
+        const float c = cose[ord[i]], sn = sine[ord[i]];
+        C4[4 * i] = c;
+        C4[4 * i + 1] = sn;
+        C4[4 * i + 2] = fabsf(c) > 1e-6f ? 1.0f / c : 0.0f;
+        C4[4 * i + 3] = fabsf(sn) > 1e-6f ? 1.0f / sn : 0.0f;
+    }
+
     GrGatherGeom g;
     g.scale = scale;
     g.L2 = L2;
@@ -84,13 +94,18 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
     g.M = M;
     g.half = half;
     g.ltbl = ltbl;
+    // the launch constants of k_grid.hip
     const float L2m = L2 + 0.01f;
     const float dmax = 3.5f * 1.41421356f + L2m * 1.41421356f + 0.05f;
+    const float rmax = g.scale * (float)(g.half - 1) * 1.0001f + dmax + 0.5f;
+    const float reach = 3.5f + L2m;
     const int nt = (M + 7) / 8;
     for (int ty = 0; ty < nt; ty++) {
         for (int tx = 0; tx < nt; tx++) {
-            int r0, r1, r2, r3;
-            gr_angle_window(tx, ty, g.fM2, dmax, akey.data(), n_ang, r0, r1, r2, r3);
+            const float cu = (float)(tx * 8) + 3.5f - g.fM2, cv = (float)(ty * 8) + 3.5f - g.fM2;
+            const bool reachable = sqrtf(cu * cu + cv * cv) <= rmax;
+            int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            if (reachable) gr_angle_window(tx, ty, g.fM2, dmax, akey.data(), n_ang, r0, r1, r2, r3);
             for (int lane = 0; lane < 64; lane++) {
                 const int a = tx * 8 + (lane & 7), b = ty * 8 + (lane >> 3);
                 if (a >= M || b >= M) continue;
@@ -101,21 +116,26 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
                 for (int part = 0; part < 2; part++) {
                     const int k0 = part ? r2 : r0, k1 = part ? r3 : r1;
                     for (int k = k0; k < k1; k++) {
-                        const int n = ord[k];
-                        const float C = cose[n], S = sine[n];
+                        const float *cs = &C4[4 * (size_t)k];
+                        if (fabsf(cu * cs[1] - cv * cs[0]) > reach * (fabsf(cs[0]) + fabsf(cs[1])) + 0.05f) continue;
+                        int td0, td1, tm0, tm1;
+                        gr_candidates_r(cu, cv, cs[2], cs[3], reach, g, td0, td1, tm0, tm1);
+                        if (td0 > td1 && tm0 > tm1) continue;
                         int jd0, jd1, jm0, jm1;
-                        gr_candidates(du, dv, C, S, L2m, g, jd0, jd1, jm0, jm1);
-                        const int cd = valid ? std::max(0, jd1 - jd0 + 1) : 0;
-                        const int cm = valid ? std::max(0, jm1 - jm0 + 1) : 0;
-                        for (int i = 0; i < cd + cm; i++) {
-                            const bool mir = i >= cd;
-                            const int j = mir ? jm0 + (i - cd) : jd0 + i;
-                            float w;
-                            if (gr_tap(j, C, S, mir ? ma : pa, mir ? mb : pb, g, wtbl, w)) {
-                                const float *v = (mir ? spec2 : spec1) + 2 * ((long)n * half + j);
-                                sr += (double)w * v[0];
-                                si += (double)w * v[1];
-                                *taps_gather += 1;
+                        gr_candidates_r(du, dv, cs[2], cs[3], L2m, g, jd0, jd1, jm0, jm1);
+                        if (!valid) continue;
+                        const int n = ord[k];
+                        for (int mir = 0; mir < 2; mir++) {
+                            const int j0 = mir ? std::max(jm0, tm0) : std::max(jd0, td0);
+                            const int j1 = mir ? std::min(jm1, tm1) : std::min(jd1, td1);
+                            for (int j = j0; j <= j1; j++) {
+                                float w;
+                                if (gr_tap(j, cs[0], cs[1], mir ? ma : pa, mir ? mb : pb, g, wtbl, w)) {
+                                    const float *v = (mir ? spec2 : spec1) + 2 * ((long)n * half + j);
+                                    sr += (double)w * v[0];
+                                    si += (double)w * v[1];
+                                    *taps_gather += 1;
+                                }
                             }
                         }
                     }
