@@ -58,6 +58,29 @@ __host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float q
     return true;
 }
 
+// The box test and the table weight factor separately per axis: for a sample
+// at coordinate X (U or V) and the grid line q of that axis, wtbl[(int)(|X-q|
+// * tblspcg + 0.5)] when ceil(X-L2) <= q <= floor(X+L2) and 1 <= q <= M-1
+// (grid.cpp:282-293), else 0.  A tap's weight is the float product of its U
+// and V factors, exactly convolv = rtmp*work[k] of grid.cpp:296; PSWF table
+// entries are positive, so a zero product means "outside the box".
+__host__ __device__ __forceinline__ float gr_axis_weight(float X, float q, const GrGatherGeom &g,
+                                                         const float *wtbl) {
+    const bool in = (X - g.L2 <= q) && (q <= X + g.L2) && (q >= 1.0f) && (q <= (float)(g.M - 1));
+    int i = gr_nn_index(fabsf(X - q) * g.tblspcg);
+    i = i > g.ltbl ? g.ltbl : (i < 0 ? 0 : i);
+    const float w = wtbl[i];
+    return in ? w : 0.0f;
+}
+
+// Sample coordinates (grid.cpp:276-277): U = (scale*j)*COSE + M2, V likewise.
+__host__ __device__ __forceinline__ void gr_sample_uv(int j, float C, float S, const GrGatherGeom &g,
+                                                      float &U, float &V) {
+    const float rt = g.scale * (float)j;
+    U = rt * C + g.fM2;
+    V = rt * S + g.fM2;
+}
+
 // Candidate samples of one angle for a cell at offset (du, dv) from the grid
 // centre: every j whose (U, V) can lie within L2 of the cell in both axes,
 // i.e. s = scale*j in {s : |s C - du| <= L2m, |s S - dv| <= L2m} with a margin

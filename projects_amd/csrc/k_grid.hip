@@ -20,9 +20,13 @@
 // centre first, so the heaviest work is dispatched first.
 //
 // Per angle, the samples that can reach the tile form one contiguous range of
-// j for the direct term and one for the mirrored term; the wave stages those
-// spectra in its own LDS slot (64 at a time) and each lane then runs its own
-// exact candidate range against LDS, so no tap waits on a global load.
+// j for the direct term and one for the mirrored term.  The box test and the
+// table weight separate by axis, so the wave evaluates them once per (sample,
+// column) and once per (sample, row) -- 2 x 8 values per sample instead of 64
+// -- and stages them with the sample spectra in its own LDS slot (64 samples
+// at a time).  Each lane then runs its own candidate range against LDS: a tap
+// is one float product wu*wv, exactly the reference's convolv, a multiply by
+// the spectrum and the compensated add.
 //
 // Accumulation: GR_ACCUM_COMPENSATED keeps a Kahan correction per component;
 // GR_ACCUM_PLAIN adds in float like the reference.
@@ -60,17 +64,44 @@ __device__ __forceinline__ void accumulate(float &sr, float &cr, float &si, floa
     }
 }
 
-// Taps of samples [j0, j1] of one angle into cell (qa, qb); spectra of samples
-// c0 .. c0+63 are staged in stage[].
+// Per-wave LDS of one chunk of <= 64 samples of one angle: their spectra, the
+// U-axis weight factor of each sample for the tile's 8 columns, the V-axis
+// factor for its 8 rows.
+struct WaveStage {
+    float2 v[64];
+    float wu[64 * 8];
+    float wv[64 * 8];
+};
+
+// Samples c0 .. c0+nj-1 of one angle, for the tile whose cells (columns qa0 +
+// da*i, rows qb0 + da*i, i < 8; da = +1 direct, -1 mirrored) the lane's cell
+// (qa0 + da*ai, qb0 + da*bi) belongs to: stage the spectra and the per-axis
+// weight factors, then accumulate the lane's taps over its candidate range
+// [j0, j1].
 template <int ACCUM>
-__device__ __forceinline__ void run_taps(int j0, int j1, int c0, float C, float S, float qa, float qb,
-                                         const GrGatherGeom &g, const float *wtbl, const float2 *stage,
-                                         float &sr, float &cr, float &si, float &ci) {
-    for (int j = j0; j <= j1; j++) {
-        float w;
-        const float2 v = stage[j - c0];
-        if (gr_tap(j, C, S, qa, qb, g, wtbl, w)) accumulate<ACCUM>(sr, cr, si, ci, w * v.x, w * v.y);
+__device__ __forceinline__ void chunk_taps(const float2 *row, int c0, int nj, int j0, int j1, float C, float S,
+                                           float qa0, float qb0, float da, int ai, int bi, int lane,
+                                           const GrGatherGeom &g, const float *wtbl, WaveStage *st, float &sr,
+                                           float &cr, float &si, float &ci) {
+    if (lane < nj) st->v[lane] = row[c0 + lane];
+    for (int e = lane; e < nj * 8; e += 64) {
+        float U, V;
+        gr_sample_uv(c0 + (e >> 3), C, S, g, U, V);
+        const float q = (float)(e & 7) * da;
+        st->wu[e] = gr_axis_weight(U, qa0 + q, g, wtbl);
+        st->wv[e] = gr_axis_weight(V, qb0 + q, g, wtbl);
     }
+    __builtin_amdgcn_wave_barrier();
+    const int lo = max(j0, c0), hi = min(j1, c0 + nj - 1);
+    for (int j = lo; j <= hi; j++) {
+        const int e = j - c0;
+        const float w = st->wu[e * 8 + ai] * st->wv[e * 8 + bi];
+        if (w != 0.0f) {
+            const float2 v = st->v[e];
+            accumulate<ACCUM>(sr, cr, si, ci, w * v.x, w * v.y);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
 }
 
 template <int ACCUM>
@@ -83,7 +114,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ti = blockIdx.x * 4 + wave;
     if (ti >= A.ntiles) return;
-    float2 *stage = reinterpret_cast<float2 *>(smem + A.lds_wtbl) + wave * 64;
+    WaveStage *st = reinterpret_cast<WaveStage *>(smem + A.lds_wtbl) + wave;
     const int p = blockIdx.y;
     const unsigned tt = A.tiles[ti];
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
@@ -92,9 +123,9 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     const float M2 = A.g.fM2;
     const float cu = (float)(tx * 8) + 3.5f - M2, cv = (float)(ty * 8) + 3.5f - M2;
 
-    const bool valid = a >= 1 && b >= 1 && a < M && b < M;
-    const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
-    const float du = pa - M2, dv = pb - M2;
+    const int ai = lane & 7, bi = lane >> 3;
+    const float du = (float)a - M2, dv = (float)b - M2;
+    const float ta = (float)(tx * 8), tb = (float)(ty * 8);   // first column / row of the tile
     const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
     float sr = 0.0f, cr = 0.0f, si = 0.0f, ci = 0.0f;
 
@@ -114,29 +145,17 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
                 if (td0 > td1 && tm0 > tm1) continue;
                 int jd0, jd1, jm0, jm1;
                 gr_candidates_r(du, dv, cs.z, cs.w, A.L2m, A.g, jd0, jd1, jm0, jm1);
-                if (!valid) {
-                    jd1 = jd0 - 1;
-                    jm1 = jm0 - 1;
-                }
                 const int n = A.aord[k];
                 const float2 *row1 = base1 + (long)n * A.half;
                 const float2 *row2 = base2 + (long)n * A.half;
-                for (int c0 = td0; c0 <= td1; c0 += 64) {
-                    const int jj = c0 + lane;
-                    stage[lane] = jj <= td1 ? row1[jj] : make_float2(0.0f, 0.0f);
-                    __builtin_amdgcn_wave_barrier();
-                    run_taps<ACCUM>(max(jd0, c0), min(min(jd1, td1), c0 + 63), c0, cs.x, cs.y, pa, pb, A.g, s_wtbl, stage,
-                                    sr, cr, si, ci);
-                    __builtin_amdgcn_wave_barrier();
-                }
-                for (int c0 = tm0; c0 <= tm1; c0 += 64) {
-                    const int jj = c0 + lane;
-                    stage[lane] = jj <= tm1 ? row2[jj] : make_float2(0.0f, 0.0f);
-                    __builtin_amdgcn_wave_barrier();
-                    run_taps<ACCUM>(max(jm0, c0), min(min(jm1, tm1), c0 + 63), c0, cs.x, cs.y, ma, mb, A.g, s_wtbl, stage,
-                                    sr, cr, si, ci);
-                    __builtin_amdgcn_wave_barrier();
-                }
+                // direct term: cell (a, b) itself
+                for (int c0 = td0; c0 <= td1; c0 += 64)
+                    chunk_taps<ACCUM>(row1, c0, min(64, td1 - c0 + 1), jd0, jd1, cs.x, cs.y, ta, tb, 1.0f, ai, bi,
+                                      lane, A.g, s_wtbl, st, sr, cr, si, ci);
+                // mirrored term: the deposit at (M-a, M-b) of grid.cpp:299-300
+                for (int c0 = tm0; c0 <= tm1; c0 += 64)
+                    chunk_taps<ACCUM>(row2, c0, min(64, tm1 - c0 + 1), jm0, jm1, cs.x, cs.y, (float)M - ta,
+                                      (float)M - tb, -1.0f, ai, bi, lane, A.g, s_wtbl, st, sr, cr, si, ci);
             }
         }
     }
@@ -175,7 +194,7 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
     A.rmax = g.scale * (float)(g.half - 1) * 1.0001f + A.dmax + 0.5f;
     const dim3 grid((t.ntiles + 3) / 4, pairs), block(256);
-    const size_t lds = sizeof(float) * A.lds_wtbl + sizeof(float2) * 4 * 64;
+    const size_t lds = sizeof(float) * A.lds_wtbl + 4 * sizeof(WaveStage);
     hipError_t e;
     if (accum == GR_ACCUM_PLAIN) {
         if ((e = gr_allow_lds(k_grid<GR_ACCUM_PLAIN>, lds)) != hipSuccess) return e;

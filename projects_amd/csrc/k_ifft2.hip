@@ -53,11 +53,20 @@ struct ColsArgs {
     long M02, out_off, out_n;
 };
 
+// Column group of a workgroup.  Workgroups are dealt to the 8 XCDs round
+// robin, each XCD with its own L2; numbering the groups so that x, x+8, x+16,
+// ... (one XCD) take neighbouring column groups lets the groups that share a
+// 128-byte line of Z fetch it through one L2 instead of four.
+__device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
+    if (ngroups % 8u) return x;
+    return (long)(x % 8u) * (ngroups / 8u) + x / 8u;
+}
+
 template <int MAXB>
 __global__ void __launch_bounds__(256) k_ifft_cols(ColsArgs A) {
     extern __shared__ float2 cols[];  // cw columns of M
     const int p = blockIdx.y;
-    const long c0 = (long)blockIdx.x * A.cw;
+    const long c0 = column_group(blockIdx.x, gridDim.x) * A.cw;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const int M = A.M, cw = A.cw;
     const float2 *zp = A.z + (long)p * M * A.out_n;

@@ -129,8 +129,15 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
                             const int j0 = mir ? std::max(jm0, tm0) : std::max(jd0, td0);
                             const int j1 = mir ? std::min(jm1, tm1) : std::min(jd1, td1);
                             for (int j = j0; j <= j1; j++) {
-                                float w;
-                                if (gr_tap(j, cs[0], cs[1], mir ? ma : pa, mir ? mb : pb, g, wtbl, w)) {
+                                // the kernel's separable form: product of per-axis factors
+                                float U, V;
+                                gr_sample_uv(j, cs[0], cs[1], g, U, V);
+                                const float w = gr_axis_weight(U, mir ? ma : pa, g, wtbl) *
+                                                gr_axis_weight(V, mir ? mb : pb, g, wtbl);
+                                float wt;
+                                const bool tap = gr_tap(j, cs[0], cs[1], mir ? ma : pa, mir ? mb : pb, g, wtbl, wt);
+                                if (tap != (w != 0.0f) || (tap && wt != w)) return -1;  // the two forms disagree
+                                if (w != 0.0f) {
                                     const float *v = (mir ? spec2 : spec1) + 2 * ((long)n * half + j);
                                     sr += (double)w * v[0];
                                     si += (double)w * v[1];

@@ -43,8 +43,9 @@ def run(g: O.Geometry, seed=0):
     tg, ts = C.c_long(0), C.c_long(0)
     f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
     d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-    _lib().gather_emul(M, half, g.n_ang, s["scale"], s["L2"], s["tblspcg"], g.ltbl, f(t["wtbl"]), f(t["cose"]),
-                       f(t["sine"]), f(spec1), f(spec2), d(Hg), d(Hs), C.byref(tg), C.byref(ts))
+    rc = _lib().gather_emul(M, half, g.n_ang, s["scale"], s["L2"], s["tblspcg"], g.ltbl, f(t["wtbl"]), f(t["cose"]),
+                            f(t["sine"]), f(spec1), f(spec2), d(Hg), d(Hs), C.byref(tg), C.byref(ts))
+    assert rc == 0, "per-axis weights and the per-tap test disagree"
     return Hg, Hs, tg.value, ts.value
 
 
