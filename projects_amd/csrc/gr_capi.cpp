@@ -296,10 +296,11 @@ int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
         }
         CK(dev_upload(pl, &pl->d_ang, ang.data(), n_ang));
     }
-    // 8x8 tiles of the M x M grid, nearest to the centre first
+    // tiles of the gridding kernel over the M x M grid, nearest to the centre first
     {
-        const int nt = (int)((g.M + 7) / 8);
-        const double c = (double)(g.M >> 1);
+        const int ts = gr_grid_tile_size();
+        const int nt = (int)((g.M + ts - 1) / ts);
+        const double c = (double)(g.M >> 1), h = 0.5 * (ts - 1);
         std::vector<unsigned> tiles;
         std::vector<double> dist;
         tiles.reserve((size_t)nt * nt);
@@ -307,7 +308,7 @@ int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
             for (int tx = 0; tx < nt; tx++) tiles.push_back((unsigned)tx | ((unsigned)ty << 16));
         dist.resize(tiles.size());
         for (size_t i = 0; i < tiles.size(); i++) {
-            const double x = (double)(tiles[i] & 0xffff) * 8 + 3.5 - c, y = (double)(tiles[i] >> 16) * 8 + 3.5 - c;
+            const double x = (double)(tiles[i] & 0xffff) * ts + h - c, y = (double)(tiles[i] >> 16) * ts + h - c;
             dist[i] = x * x + y * y;
         }
         std::vector<size_t> idx(tiles.size());

@@ -225,6 +225,69 @@ __host__ __device__ __forceinline__ void gr_kahan(float &s, float &c, float x) {
     s = t;
 }
 
+// ---------------------------------------------------------------------------
+// Tile-local scatter form (the gridding kernel): a wavefront owns a tile of
+// cells and walks the samples whose boxes reach it, one lane per (sample,
+// box cell).  Cell (iul + du, ivl + dv), du, dv < bw, of the sample's box
+// [iul, iuh] x [ivl, ivh] (iul = ceil(U-L2), iuh = floor(U+L2), clipped to
+// [1, M-1], grid.cpp:282-285); bw = floor(L) + 1 covers every box.
+// ---------------------------------------------------------------------------
+
+// Angle window for a tile centred at (cu, cv) from the grid centre: ranges
+// [r0, r1) and [r2, r3) of sorted keys whose line passes within dmax.
+__host__ __device__ __forceinline__ void gr_angle_window_at(float cu, float cv, float dmax, const float *akey,
+                                                            int n_ang, int &r0, int &r1, int &r2, int &r3) {
+    const float Rt = sqrtf(cu * cu + cv * cv);
+    r0 = 0;
+    r1 = n_ang;
+    r2 = 0;
+    r3 = 0;
+    if (Rt <= dmax + 1.0f) return;
+    const float kPi = 3.14159265f;
+    float phi = atan2f(cv, cu);
+    if (phi < 0.0f) phi += kPi;
+    if (phi >= kPi) phi -= kPi;
+    const float alpha = asinf(fminf(1.0f, dmax / Rt)) + 1e-4f;
+    const float lo = phi - alpha, hi = phi + alpha;
+    if (lo < 0.0f) {
+        r0 = gr_lower_bound(akey, n_ang, lo + kPi);
+        r3 = gr_upper_bound(akey, n_ang, hi);
+    } else if (hi >= kPi) {
+        r0 = gr_lower_bound(akey, n_ang, lo);
+        r3 = gr_upper_bound(akey, n_ang, hi - kPi);
+    } else {
+        r0 = gr_lower_bound(akey, n_ang, lo);
+        r1 = gr_upper_bound(akey, n_ang, hi);
+    }
+}
+
+// Box cell (du, dv) of sample j: its grid cell (iu, iv) and tap weight, or
+// false when the cell is outside the box or the grid interior.
+__host__ __device__ __forceinline__ bool gr_box_tap(int j, float C, float S, int du, int dv,
+                                                    const GrGatherGeom &g, const float *wtbl, int &iu,
+                                                    int &iv, float &w) {
+    float U, V;
+    gr_sample_uv(j, C, S, g, U, V);
+    const float ulo = U - g.L2, uhi = U + g.L2, vlo = V - g.L2, vhi = V + g.L2;
+    iu = (int)ceilf(ulo) + du;
+    iv = (int)ceilf(vlo) + dv;
+    const float fu = (float)iu, fv = (float)iv;
+    if (!(fu <= uhi && fv <= vhi && iu >= 1 && iv >= 1 && iu <= g.M - 1 && iv <= g.M - 1)) return false;
+    int ku = gr_nn_index(fabsf(U - fu) * g.tblspcg);
+    int kv = gr_nn_index(fabsf(V - fv) * g.tblspcg);
+    ku = ku > g.ltbl ? g.ltbl : ku;
+    kv = kv > g.ltbl ? g.ltbl : kv;
+    w = wtbl[ku] * wtbl[kv];  // convolv = rtmp*work[k], grid.cpp:296
+    return true;
+}
+
+// Samples of one angle closer than this many steps of j can have overlapping
+// boxes; samples K apart cannot: their coordinates differ by K*scale times
+// max(|C|, |S|) >= 0.7071 in one axis, which must exceed the box width L.
+static inline int gr_conflict_free_step(float L, float scale) {
+    return (int)floorf((L + 0.01f) * 1.41422f / scale) + 1;
+}
+
 // Error-free transformation (Knuth's TwoSum): s + x = t + e exactly.
 __host__ __device__ __forceinline__ void gr_two_sum(float &s, float &c, float x) {
     const float t = s + x;

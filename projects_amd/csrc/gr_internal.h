@@ -56,7 +56,9 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
                             long slice_stride, long row_stride, int n_slices, int pair0, int pairs,
                             float2 *spec1, float2 *spec2, hipStream_t s);
 
-// K2: gather gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
+// K2: tile-local gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
+// The plan lists the kernel's tiles (gr_grid_tile_size() cells per side).
+int gr_grid_tile_size();
 hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
                           const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
                           int accum, hipStream_t s);
