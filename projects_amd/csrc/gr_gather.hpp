@@ -41,14 +41,21 @@ __host__ __device__ __forceinline__ int gr_nn_index(float X) {
     return i + (f >= 0.5f ? 1 : 0);
 }
 
+// Sample coordinates (grid.cpp:276-277): U = (scale*j)*COSE + M2, V likewise.
+__host__ __device__ __forceinline__ void gr_sample_uv(int j, float C, float S, const GrGatherGeom &g,
+                                                      float &U, float &V) {
+    const float rt = g.scale * (float)j;
+    U = rt * C + g.fM2;
+    V = rt * S + g.fM2;
+}
+
 // If sample j of the angle with table values (C, S) deposits into cell
 // (qa, qb), return true and its weight.  qa, qb are exact small integers.
 __host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float qa, float qb,
                                                 const GrGatherGeom &g, const float *wtbl,
                                                 float &w) {
-    const float rt = g.scale * (float)j;
-    const float U = rt * C + g.fM2;
-    const float V = rt * S + g.fM2;
+    float U, V;
+    gr_sample_uv(j, C, S, g, U, V);
     if (!(U - g.L2 <= qa && qa <= U + g.L2 && V - g.L2 <= qb && qb <= V + g.L2)) return false;
     int iu = gr_nn_index(fabsf(U - qa) * g.tblspcg);
     int iv = gr_nn_index(fabsf(V - qb) * g.tblspcg);
@@ -58,74 +65,6 @@ __host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float q
     return true;
 }
 
-// The box test and the table weight factor separately per axis: for a sample
-// at coordinate X (U or V) and the grid line q of that axis, wtbl[(int)(|X-q|
-// * tblspcg + 0.5)] when ceil(X-L2) <= q <= floor(X+L2) and 1 <= q <= M-1
-// (grid.cpp:282-293), else 0.  A tap's weight is the float product of its U
-// and V factors, exactly convolv = rtmp*work[k] of grid.cpp:296; PSWF table
-// entries are positive, so a zero product means "outside the box".
-__host__ __device__ __forceinline__ float gr_axis_weight(float X, float q, const GrGatherGeom &g,
-                                                         const float *wtbl) {
-    const bool in = (X - g.L2 <= q) && (q <= X + g.L2) && (q >= 1.0f) && (q <= (float)(g.M - 1));
-    int i = gr_nn_index(fabsf(X - q) * g.tblspcg);
-    i = i > g.ltbl ? g.ltbl : (i < 0 ? 0 : i);
-    const float w = wtbl[i];
-    return in ? w : 0.0f;
-}
-
-// Sample coordinates (grid.cpp:276-277): U = (scale*j)*COSE + M2, V likewise.
-__host__ __device__ __forceinline__ void gr_sample_uv(int j, float C, float S, const GrGatherGeom &g,
-                                                      float &U, float &V) {
-    const float rt = g.scale * (float)j;
-    U = rt * C + g.fM2;
-    V = rt * S + g.fM2;
-}
-
-// Candidate samples of one angle for a cell at offset (du, dv) from the grid
-// centre: every j whose (U, V) can lie within L2 of the cell in both axes,
-// i.e. s = scale*j in {s : |s C - du| <= L2m, |s S - dv| <= L2m} with a margin
-// L2m > L2 covering float rounding of U, V.  The mirrored cell (M-a, M-b) sits
-// at (-du, -dv), whose interval is the negation of this one.  Returns the
-// direct range [jd0, jd1] and the mirrored range [jm0, jm1] (empty when
-// j0 > j1), both clipped to 1 .. half-1.
-__host__ __device__ __forceinline__ void gr_candidates(float du, float dv, float C, float S,
-                                                       float L2m, const GrGatherGeom &g,
-                                                       int &jd0, int &jd1, int &jm0, int &jm1) {
-    float lo = -1e9f, hi = 1e9f;
-    bool empty = false;
-    if (fabsf(C) > 1e-6f) {
-        const float ic = 1.0f / C;
-        const float x0 = (du - L2m) * ic, x1 = (du + L2m) * ic;
-        lo = fmaxf(lo, fminf(x0, x1));
-        hi = fminf(hi, fmaxf(x0, x1));
-    } else if (fabsf(du) > L2m) {
-        empty = true;
-    }
-    if (fabsf(S) > 1e-6f) {
-        const float is = 1.0f / S;
-        const float y0 = (dv - L2m) * is, y1 = (dv + L2m) * is;
-        lo = fmaxf(lo, fminf(y0, y1));
-        hi = fminf(hi, fmaxf(y0, y1));
-    } else if (fabsf(dv) > L2m) {
-        empty = true;
-    }
-    lo = fmaxf(lo, -1e9f);
-    hi = fminf(hi, 1e9f);
-    const float inv = 1.0f / g.scale;
-    const float jmax = (float)(g.half - 1);
-    if (empty || lo > hi || jmax < 1.0f) {
-        jd0 = jm0 = 1;
-        jd1 = jm1 = 0;
-        return;
-    }
-    // direct: s in [lo, hi]; mirrored: s in [-hi, -lo]; s > 0 only
-    const float d0 = fmaxf(1.0f, ceilf(lo * inv - 1e-3f));
-    const float d1 = fminf(jmax, floorf(hi * inv + 1e-3f));
-    const float m0 = fmaxf(1.0f, ceilf(-hi * inv - 1e-3f));
-    const float m1 = fminf(jmax, floorf(-lo * inv + 1e-3f));
-    if (d0 <= d1) { jd0 = (int)d0; jd1 = (int)d1; } else { jd0 = 1; jd1 = 0; }
-    if (m0 <= m1) { jm0 = (int)m0; jm1 = (int)m1; } else { jm0 = 1; jm1 = 0; }
-}
 
 // Angle window of an 8 x 8 tile with origin (tx*8, ty*8): the ranges
 // [r0, r1) and [r2, r3) of sorted angle keys (akey, ascending in [0, pi)) whose
@@ -179,9 +118,15 @@ __host__ __device__ __forceinline__ void gr_angle_window(int tx, int ty, float M
     }
 }
 
-// gr_candidates with precomputed reciprocals: ic = 1/C and is = 1/S, or 0 when
-// |C| (|S|) <= 1e-6, where that axis gives no bound on s (the samples stay
-// within 2048*1e-6 of the centre line) and the cell must lie within L2m of it.
+// Candidate samples of one angle for a cell at offset (du, dv) from the grid
+// centre: every j whose (U, V) can lie within L2 of the cell in both axes,
+// i.e. s = scale*j in {s : |s C - du| <= L2m, |s S - dv| <= L2m} with a margin
+// L2m > L2 covering float rounding of U, V.  ic = 1/C and is = 1/S, or 0 when
+// |C| (|S|) <= 1e-6: that axis then bounds nothing (the samples stay within
+// 2048*1e-6 of the centre line) and the cell must lie within L2m of it.  The
+// mirrored cell (M-a, M-b) sits at (-du, -dv), whose interval is the negation
+// of this one.  Returns the direct range [jd0, jd1] and the mirrored range
+// [jm0, jm1] (empty when j0 > j1), both clipped to 1 .. half-1.
 __host__ __device__ __forceinline__ void gr_candidates_r(float du, float dv, float ic, float is,
                                                          float L2m, const GrGatherGeom &g,
                                                          int &jd0, int &jd1, int &jm0, int &jm1) {
@@ -223,76 +168,4 @@ __host__ __device__ __forceinline__ void gr_kahan(float &s, float &c, float x) {
     const float t = s + y;
     c = (t - s) - y;
     s = t;
-}
-
-// ---------------------------------------------------------------------------
-// Tile-local scatter form (the gridding kernel): a wavefront owns a tile of
-// cells and walks the samples whose boxes reach it, one lane per (sample,
-// box cell).  Cell (iul + du, ivl + dv), du, dv < bw, of the sample's box
-// [iul, iuh] x [ivl, ivh] (iul = ceil(U-L2), iuh = floor(U+L2), clipped to
-// [1, M-1], grid.cpp:282-285); bw = floor(L) + 1 covers every box.
-// ---------------------------------------------------------------------------
-
-// Angle window for a tile centred at (cu, cv) from the grid centre: ranges
-// [r0, r1) and [r2, r3) of sorted keys whose line passes within dmax.
-__host__ __device__ __forceinline__ void gr_angle_window_at(float cu, float cv, float dmax, const float *akey,
-                                                            int n_ang, int &r0, int &r1, int &r2, int &r3) {
-    const float Rt = sqrtf(cu * cu + cv * cv);
-    r0 = 0;
-    r1 = n_ang;
-    r2 = 0;
-    r3 = 0;
-    if (Rt <= dmax + 1.0f) return;
-    const float kPi = 3.14159265f;
-    float phi = atan2f(cv, cu);
-    if (phi < 0.0f) phi += kPi;
-    if (phi >= kPi) phi -= kPi;
-    const float alpha = asinf(fminf(1.0f, dmax / Rt)) + 1e-4f;
-    const float lo = phi - alpha, hi = phi + alpha;
-    if (lo < 0.0f) {
-        r0 = gr_lower_bound(akey, n_ang, lo + kPi);
-        r3 = gr_upper_bound(akey, n_ang, hi);
-    } else if (hi >= kPi) {
-        r0 = gr_lower_bound(akey, n_ang, lo);
-        r3 = gr_upper_bound(akey, n_ang, hi - kPi);
-    } else {
-        r0 = gr_lower_bound(akey, n_ang, lo);
-        r1 = gr_upper_bound(akey, n_ang, hi);
-    }
-}
-
-// Box cell (du, dv) of sample j: its grid cell (iu, iv) and tap weight, or
-// false when the cell is outside the box or the grid interior.
-__host__ __device__ __forceinline__ bool gr_box_tap(int j, float C, float S, int du, int dv,
-                                                    const GrGatherGeom &g, const float *wtbl, int &iu,
-                                                    int &iv, float &w) {
-    float U, V;
-    gr_sample_uv(j, C, S, g, U, V);
-    const float ulo = U - g.L2, uhi = U + g.L2, vlo = V - g.L2, vhi = V + g.L2;
-    iu = (int)ceilf(ulo) + du;
-    iv = (int)ceilf(vlo) + dv;
-    const float fu = (float)iu, fv = (float)iv;
-    if (!(fu <= uhi && fv <= vhi && iu >= 1 && iv >= 1 && iu <= g.M - 1 && iv <= g.M - 1)) return false;
-    int ku = gr_nn_index(fabsf(U - fu) * g.tblspcg);
-    int kv = gr_nn_index(fabsf(V - fv) * g.tblspcg);
-    ku = ku > g.ltbl ? g.ltbl : ku;
-    kv = kv > g.ltbl ? g.ltbl : kv;
-    w = wtbl[ku] * wtbl[kv];  // convolv = rtmp*work[k], grid.cpp:296
-    return true;
-}
-
-// Samples of one angle closer than this many steps of j can have overlapping
-// boxes; samples K apart cannot: their coordinates differ by K*scale times
-// max(|C|, |S|) >= 0.7071 in one axis, which must exceed the box width L.
-static inline int gr_conflict_free_step(float L, float scale) {
-    return (int)floorf((L + 0.01f) * 1.41422f / scale) + 1;
-}
-
-// Error-free transformation (Knuth's TwoSum): s + x = t + e exactly.
-__host__ __device__ __forceinline__ void gr_two_sum(float &s, float &c, float x) {
-    const float t = s + x;
-    const float bp = t - s;
-    const float e = (s - (t - bp)) + (x - bp);
-    s = t;
-    c += e;
 }

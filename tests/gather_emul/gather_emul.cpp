@@ -95,71 +95,53 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
     g.half = half;
     g.ltbl = ltbl;
     // the launch constants of k_grid.hip
-    const int TS = 16;
-    const float L = 2.0f * L2;
-    const int K = gr_conflict_free_step(L, scale);
-    const int bw = (int)floorf(L + 0.01f) + 1;
-    const int G = bw * bw <= 16 ? 16 : (bw * bw <= 32 ? 32 : 64), SP = 64 / G;
     const float L2m = L2 + 0.01f;
-    const float half_tile = 0.5f * (float)(TS - 1);
-    const float reach = half_tile + L2m;
-    const float dmax = reach * 1.41421356f + 0.05f;
+    const float dmax = 3.5f * 1.41421356f + L2m * 1.41421356f + 0.05f;
     const float rmax = g.scale * (float)(g.half - 1) * 1.0001f + dmax + 0.5f;
-    const int nt = (M + TS - 1) / TS;
-    std::vector<int> stamp((size_t)TS * TS, -1);
-    int pass_id = 0;
+    const float reach = 3.5f + L2m;
+    const int nt = (M + 7) / 8;
     for (int ty = 0; ty < nt; ty++) {
         for (int tx = 0; tx < nt; tx++) {
-            const int a0 = tx * TS, b0 = ty * TS;
-            const float cu = (float)a0 + half_tile - g.fM2, cv = (float)b0 + half_tile - g.fM2;
-            if (sqrtf(cu * cu + cv * cv) > rmax) continue;
-            int r0, r1, r2, r3;
-            gr_angle_window_at(cu, cv, dmax, akey.data(), n_ang, r0, r1, r2, r3);
-            for (int part = 0; part < 2; part++) {
-                for (int k = part ? r2 : r0; k < (part ? r3 : r1); k++) {
-                    const float *cs = &C4[4 * (size_t)k];
-                    if (fabsf(cu * cs[1] - cv * cs[0]) > reach * (fabsf(cs[0]) + fabsf(cs[1])) + 0.05f) continue;
-                    int td0, td1, tm0, tm1;
-                    gr_candidates_r(cu, cv, cs[2], cs[3], reach, g, td0, td1, tm0, tm1);
-                    if (td0 > td1 && tm0 > tm1) continue;
-                    const int n = ord[k];
-                    for (int mirror = 0; mirror < 2; mirror++) {
-                        const int t0 = mirror ? tm0 : td0, t1 = mirror ? tm1 : td1;
-                        const float *row = (mirror ? spec2 : spec1) + 2 * (long)n * half;
-                        for (int c0 = t0; c0 <= t1; c0 += 64) {
-                            const int nj = std::min(64, t1 - c0 + 1);
-                            for (int r = 0; r < K && r < nj; r++) {
-                                const int cnt = (nj - r + K - 1) / K;
-                                for (int q0 = 0; q0 < cnt; q0 += SP) {
-                                    pass_id++;
-                                    for (int lane = 0; lane < 64; lane++) {
-                                        const int slot = lane / G, sub = lane - slot * G;
-                                        const int q = q0 + slot;
-                                        if (q >= cnt) continue;
-                                        const int jl = r + K * q;
-                                        for (int t = sub; t < bw * bw; t += G) {
-                                            const int dv = t / bw, du = t - dv * bw;
-                                            int iu, iv;
-                                            float w;
-                                            if (!gr_box_tap(c0 + jl, cs[0], cs[1], du, dv, g, wtbl, iu, iv, w)) continue;
-                                            const int ca = (mirror ? M - iu : iu) - a0, cb = (mirror ? M - iv : iv) - b0;
-                                            if (ca < 0 || ca >= TS || cb < 0 || cb >= TS) continue;
-                                            // two lanes of one pass must never share a cell
-                                            int &st = stamp[(size_t)cb * TS + ca];
-                                            if (st == pass_id) return -2;
-                                            st = pass_id;
-                                            const float *v = row + 2 * (c0 + jl);
-                                            double *h = H_gather + 2 * ((long)(a0 + ca) * M + (b0 + cb));
-                                            h[0] += (double)w * v[0];
-                                            h[1] += (double)w * v[1];
-                                            *taps_gather += 1;
-                                        }
-                                    }
+            const float cu = (float)(tx * 8) + 3.5f - g.fM2, cv = (float)(ty * 8) + 3.5f - g.fM2;
+            const bool reachable = sqrtf(cu * cu + cv * cv) <= rmax;
+            int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            if (reachable) gr_angle_window(tx, ty, g.fM2, dmax, akey.data(), n_ang, r0, r1, r2, r3);
+            for (int lane = 0; lane < 64; lane++) {
+                const int a = tx * 8 + (lane & 7), b = ty * 8 + (lane >> 3);
+                if (a >= M || b >= M) continue;
+                const bool valid = a >= 1 && b >= 1;
+                const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
+                const float du = pa - g.fM2, dv = pb - g.fM2;
+                double sr = 0, si = 0;
+                for (int part = 0; part < 2; part++) {
+                    const int k0 = part ? r2 : r0, k1 = part ? r3 : r1;
+                    for (int k = k0; k < k1; k++) {
+                        const float *cs = &C4[4 * (size_t)k];
+                        if (fabsf(cu * cs[1] - cv * cs[0]) > reach * (fabsf(cs[0]) + fabsf(cs[1])) + 0.05f) continue;
+                        int td0, td1, tm0, tm1;
+                        gr_candidates_r(cu, cv, cs[2], cs[3], reach, g, td0, td1, tm0, tm1);
+                        if (td0 > td1 && tm0 > tm1) continue;
+                        int jd0, jd1, jm0, jm1;
+                        gr_candidates_r(du, dv, cs[2], cs[3], L2m, g, jd0, jd1, jm0, jm1);
+                        if (!valid) continue;
+                        const int n = ord[k];
+                        for (int mir = 0; mir < 2; mir++) {
+                            const int j0 = mir ? std::max(jm0, tm0) : std::max(jd0, td0);
+                            const int j1 = mir ? std::min(jm1, tm1) : std::min(jd1, td1);
+                            for (int j = j0; j <= j1; j++) {
+                                float w;
+                                if (gr_tap(j, cs[0], cs[1], mir ? ma : pa, mir ? mb : pb, g, wtbl, w)) {
+                                    const float *v = (mir ? spec2 : spec1) + 2 * ((long)n * half + j);
+                                    sr += (double)w * v[0];
+                                    si += (double)w * v[1];
+                                    *taps_gather += 1;
                                 }
                             }
                         }
                     }
                 }
+                H_gather[2 * ((long)a * M + b)] = sr;
+                H_gather[2 * ((long)a * M + b) + 1] = si;
             }
         }
     }
