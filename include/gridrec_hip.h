@@ -166,12 +166,13 @@ int gr_get_timing(gr_plan *plan, gr_timing *out, int reset);
 /* Synthetic sinograms on the device (the generator of projects_amd/phantom.py):
  * n_slices slices [slice0, slice0+n_slices) of seed, each n_ang x n_det floats,
  * n_ell ellipses; photons > 0 adds Poisson noise and -log; density 0 is
- * U[-0.3,1], 1 is log-uniform over 4 decades.  centers[s] is the axis of slice
+ * U[-0.3,1], 1 is log-uniform over 4 decades; gauss_rel > 0 multiplies each
+ * value by (1 + gauss_rel * z), z ~ N(0,1).  centers[s] is the axis of slice
  * s (n_slices floats, host). */
 int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0,
                          unsigned long long seed, int n_ang, int n_det,
                          const float *centers, int n_ell, float photons, int density,
-                         void *stream);
+                         float gauss_rel, void *stream);
 
 const char *gr_last_error(void);
 const char *gr_version(void);

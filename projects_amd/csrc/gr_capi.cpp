@@ -211,7 +211,7 @@ int gr_filter_lookup(const char *name, char *name_io, int *filter_id) {
 }
 
 int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
-    if (!plan_out) return fail(GR_EINVAL, "null plan pointer");
+    if (!plan_out || !params) return fail(GR_EINVAL, "null argument");
     *plan_out = nullptr;
     const char *why = "";
     gr_plan *pl = new gr_plan();
@@ -467,7 +467,7 @@ int gr_get_timing(gr_plan *pl, gr_timing *out, int reset) {
 
 int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0, unsigned long long seed,
                          int n_ang, int n_det, const float *centers, int n_ell, float photons, int density,
-                         void *stream) {
+                         float gauss_rel, void *stream) {
     if (!sino || !centers || n_slices < 1 || n_ang < 1 || n_det < 1 || n_ell < 1 || n_ell > 64)
         return fail(GR_EINVAL, "invalid phantom arguments (1 <= n_ell <= 64)");
     if (slice_stride < (long)n_ang * n_det) return fail(GR_EINVAL, "slice stride too small");
@@ -476,7 +476,7 @@ int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slic
     GR_HIP(hipMallocAsync((void **)&dc, sizeof(float) * n_slices, s));
     GR_HIP(hipMemcpyAsync(dc, centers, sizeof(float) * n_slices, hipMemcpyHostToDevice, s));
     hipError_t e = gr_launch_phantom(sino, slice_stride, n_slices, slice0, seed, n_ang, n_det, dc, n_ell, photons,
-                                     density, s);
+                                     density, gauss_rel, s);
     hipError_t f = hipFreeAsync(dc, s);
     hipError_t w = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "phantom kernels");

@@ -107,7 +107,7 @@ struct SinoArgs {
     const float *centers;
     long slice_stride, slice0;
     int n_ang, n_det, n_ell;
-    float photons;
+    float photons, gauss_rel;
     unsigned k0, k1;
 };
 
@@ -138,6 +138,11 @@ __global__ void __launch_bounds__(256) k_sinogram(SinoArgs A) {
                                      (unsigned)(A.slice0 + s), A.k0, A.k1);
         p = -logf(fmaxf(I, 1.0f) / A.photons);
     }
+    if (A.gauss_rel > 0.0f) {  // p * (1 + gauss_rel * z), z by Box-Muller on counter word 1<<16
+        const Philox w = philox4x32_10((unsigned)k, (unsigned)n, (unsigned)(A.slice0 + s), 1u << 16, A.k0, A.k1);
+        const float z = sqrtf(-2.0f * logf(u01(w.x[0]))) * cospif(2.0f * u01(w.x[1]));
+        p = p * (1.0f + A.gauss_rel * z);
+    }
     A.sino[(long)s * A.slice_stride + (long)n * A.n_det + k] = p;
 }
 
@@ -145,7 +150,7 @@ __global__ void __launch_bounds__(256) k_sinogram(SinoArgs A) {
 
 hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long slice0,
                              unsigned long long seed, int n_ang, int n_det, const float *centers_dev,
-                             int n_ell, float photons, int density, hipStream_t s) {
+                             int n_ell, float photons, int density, float gauss_rel, hipStream_t s) {
     if (n_ell < 1 || n_ell > 64 || n_slices < 1 || n_ang < 1 || n_det < 1) return hipErrorInvalidValue;
     GrEllipse *E = nullptr;
     hipError_t e = hipMallocAsync((void **)&E, sizeof(GrEllipse) * (size_t)n_slices * n_ell, s);
@@ -153,7 +158,7 @@ hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long 
     const unsigned k0 = (unsigned)(seed & 0xffffffffu), k1 = (unsigned)(seed >> 32);
     hipLaunchKernelGGL(k_ellipses, dim3((n_ell + 63) / 64, n_slices), dim3(64), 0, s, E, n_slices, slice0,
                        n_ell, k0, k1, density);
-    SinoArgs A{sino, E, centers_dev, slice_stride, slice0, n_ang, n_det, n_ell, photons, k0, k1};
+    SinoArgs A{sino, E, centers_dev, slice_stride, slice0, n_ang, n_det, n_ell, photons, gauss_rel, k0, k1};
     for (int s0 = 0; s0 < n_slices; s0 += 65535) {
         const int ns = n_slices - s0 < 65535 ? n_slices - s0 : 65535;
         SinoArgs B = A;

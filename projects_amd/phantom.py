@@ -212,26 +212,31 @@ def poisson(lam: np.ndarray, seed: int, slice_idx: int, max_rounds: int = 64) ->
     return out
 
 
+def gaussian(seed: int, slice_idx: int, n_ang: int, n_det: int) -> np.ndarray:
+    """Standard normal draws per cell: Box-Muller on words 0-1 of counter
+    (k, n, slice, 1 << 16) (a counter word no Poisson round reaches)."""
+    k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+    kk, nn = np.meshgrid(np.arange(n_det, dtype=np.uint32), np.arange(n_ang, dtype=np.uint32))
+    w = philox4x32(kk, nn, slice_idx, 1 << 16, k0, k1)
+    u1, u2 = u01(w[0]), u01(w[1])
+    return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
+
+
 def sinogram(seed: int, slice_idx: int, n_ang: int, n_det: int, center: float, n_ell: int,
-             photons: float = 0.0, density: int = DENSITY_UNIFORM, gauss_sigma: float = 0.0) -> np.ndarray:
+             photons: float = 0.0, density: int = DENSITY_UNIFORM, gauss_rel: float = 0.0) -> np.ndarray:
     """One float32 sinogram (n_ang x n_det).
 
     photons > 0: I ~ Poisson(photons * exp(-p)), value -log(max(I,1)/photons).
-    gauss_sigma > 0: additive Gaussian noise of gauss_sigma * max|p| (Box-Muller
-    on the noise counters, words 0-1), used by the precision study.
+    gauss_rel > 0: multiplicative Gaussian noise, p * (1 + gauss_rel * z) with
+    z ~ N(0,1) per cell (the precision study's "1% Gaussian noise").
     """
     E = ellipses(seed, slice_idx, n_ell, density)
     p = radon_ellipses(E, n_ang, n_det, center)
     if photons > 0:
         I = poisson(photons * np.exp(-p), seed, slice_idx)
         p = -np.log(np.maximum(I, 1.0) / photons)
-    if gauss_sigma > 0:
-        k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
-        kk, nn = np.meshgrid(np.arange(n_det, dtype=np.uint32), np.arange(n_ang, dtype=np.uint32))
-        w = philox4x32(kk, nn, slice_idx, 2, k0, k1)
-        u1, u2 = u01(w[0]), u01(w[1])
-        z = np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)
-        p = p + gauss_sigma * float(np.max(np.abs(p))) * z
+    if gauss_rel > 0:
+        p = p * (1.0 + gauss_rel * gaussian(seed, slice_idx, n_ang, n_det))
     return p.astype(np.float32)
 
 
