@@ -17,7 +17,7 @@
 #include <string>
 #include <vector>
 
-#include "gr_internal.h"
+#include "gr_plan.h"
 
 namespace {
 
@@ -41,53 +41,6 @@ int hip_fail(hipError_t e, const char *what) {
 
 }  // namespace
 
-struct gr_plan {
-    gr_params prm;
-    GrGeom g;
-    std::vector<float> angles;  // copy of the caller's list (geom 0)
-    hipStream_t stream = nullptr;
-    // device tables
-    float *d_cose = nullptr, *d_sine = nullptr, *d_roi = nullptr, *d_wtbl = nullptr, *d_winv = nullptr;
-    float2 *d_filphase = nullptr, *d_tw_row = nullptr, *d_tw_grid = nullptr;
-    float *d_akey = nullptr;
-    int *d_aord = nullptr;
-    float4 *d_ang = nullptr;
-    unsigned *d_tiles = nullptr;
-    int ntiles = 0;
-    // workspace
-    int max_pairs = 1;
-    float2 *d_spec1 = nullptr, *d_spec2 = nullptr, *d_ht = nullptr, *d_z = nullptr;
-    size_t dev_bytes = 0;
-    // host staging for gr_recon_host
-    float *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr;
-    // timing: event pairs recorded around launches, resolved in gr_get_timing
-    int timing = 0;
-    gr_timing t{};
-    struct Pending {
-        int kind;  // 0 rowfft, 1 grid, 2 ifft rows, 3 ifft cols
-        hipEvent_t e0, e1;
-    };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> event_pool;
-
-    GrDevTables tables() const {
-        GrDevTables t;
-        t.cose = d_cose;
-        t.sine = d_sine;
-        t.roi_offset = d_roi;
-        t.filphase = d_filphase;
-        t.wtbl = d_wtbl;
-        t.winv = d_winv;
-        t.tw_row = d_tw_row;
-        t.tw_grid = d_tw_grid;
-        t.akey = d_akey;
-        t.aord = d_aord;
-        t.ang = d_ang;
-        t.tiles = d_tiles;
-        t.ntiles = ntiles;
-        return t;
-    }
-};
 
 namespace {
 
@@ -110,10 +63,14 @@ int dev_alloc(gr_plan *pl, T **dst, size_t n) {
 void free_plan(gr_plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase,
-                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_ang, pl->d_tiles, pl->d_spec1,
-                    pl->d_spec2, pl->d_ht, pl->d_z, pl->d_in, pl->d_out};
+                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_ang, pl->d_tiles, pl->d_in, pl->d_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (pl->owns_workspace) {
+        void *ws[] = {pl->d_spec1, pl->d_spec2, pl->d_ht, pl->d_z};
+        for (void *p : ws)
+            if (p) (void)hipFree(p);
+    }
     if (pl->h_in) (void)hipHostFree(pl->h_in);
     if (pl->h_out) (void)hipHostFree(pl->h_out);
     for (auto &q : pl->pending) {
@@ -210,7 +167,38 @@ int gr_filter_lookup(const char *name, char *name_io, int *filter_id) {
     return GR_OK;
 }
 
-int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
+}  // extern "C"
+
+int gr_set_error(int code, const char *msg) { return fail(code, msg); }
+
+size_t gr_plan_pair_bytes(const gr_plan *pl) {
+    const GrGeom &g = pl->g;
+    const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    return 2 * up(sizeof(float2) * spec_pp) + up(sizeof(float2) * ht_pp) + up(sizeof(float2) * z_pp);
+}
+
+void gr_plan_bind_workspace(gr_plan *pl, void *base, int max_pairs) {
+    const GrGeom &g = pl->g;
+    const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *p = (char *)base;
+    pl->d_spec1 = (float2 *)p;
+    p += up(sizeof(float2) * spec_pp * max_pairs);
+    pl->d_spec2 = (float2 *)p;
+    p += up(sizeof(float2) * spec_pp * max_pairs);
+    pl->d_ht = (float2 *)p;
+    p += up(sizeof(float2) * ht_pp * max_pairs);
+    pl->d_z = (float2 *)p;
+    pl->max_pairs = max_pairs;
+    pl->owns_workspace = false;
+}
+
+extern "C" int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
+    return gr_plan_create_impl(params, plan_out, true);
+}
+
+int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_workspace) {
     if (!plan_out || !params) return fail(GR_EINVAL, "null argument");
     *plan_out = nullptr;
     const char *why = "";
@@ -328,14 +316,20 @@ int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
         mp = (int)std::max<size_t>(1, std::min<size_t>(16, budget / per_pair));
     }
     pl->max_pairs = mp;
-    CK(dev_alloc(pl, &pl->d_spec1, spec_pp * mp));
-    CK(dev_alloc(pl, &pl->d_spec2, spec_pp * mp));
-    CK(dev_alloc(pl, &pl->d_ht, ht_pp * mp));
-    CK(dev_alloc(pl, &pl->d_z, z_pp * mp));
+    if (alloc_workspace) {
+        CK(dev_alloc(pl, &pl->d_spec1, spec_pp * mp));
+        CK(dev_alloc(pl, &pl->d_spec2, spec_pp * mp));
+        CK(dev_alloc(pl, &pl->d_ht, ht_pp * mp));
+        CK(dev_alloc(pl, &pl->d_z, z_pp * mp));
+    } else {
+        pl->owns_workspace = false;
+    }
 #undef CK
     *plan_out = pl;
     return GR_OK;
 }
+
+extern "C" {
 
 int gr_plan_info(const gr_plan *pl, gr_info *info) {
     if (!pl || !info) return fail(GR_EINVAL, "null argument");
@@ -373,6 +367,7 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
     if (!pl || !sino || !out) return fail(GR_EINVAL, "null argument");
     if (n_slices < 0) return fail(GR_EINVAL, "negative slice count");
     if (n_slices == 0) return GR_OK;
+    if (!pl->d_spec1) return fail(GR_EINVAL, "plan has no workspace");
     const GrGeom &g = pl->g;
     if (sino_row_stride < g.n_det_in || sino_slice_stride < sino_row_stride * (long)pl->prm.n_ang)
         return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
@@ -403,7 +398,8 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
         }
         {
             KernelTimer kt(pl, s, 3);
-            GR_HIP(gr_launch_ifft_cols(g, t, pl->d_z, out, out_slice_stride, out_row_stride, n_slices, p0, np, s));
+            GR_HIP(gr_launch_ifft_cols(g, t, pl->d_z, out, out_slice_stride, out_row_stride, n_slices, p0, np,
+                                       pl->cols_per_block, s));
             kt.done();
         }
         if (pl->timing) pl->t.pairs += np;

@@ -73,7 +73,7 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
 // unless that slice index is >= n_slices.
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
                                long out_slice_stride, long out_row_stride, int n_slices, int pair0,
-                               int pairs, hipStream_t s);
+                               int pairs, int cols_per_block, hipStream_t s);
 
 // FP64 tables on the device, rounded to float: filphase (half), wtbl (ltbl+1),
 // winv (M0).

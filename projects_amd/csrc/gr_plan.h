@@ -1,0 +1,68 @@
+// gr_plan.h -- the plan object behind the C-ABI, shared by gr_capi.cpp (plans)
+// and gr_queue.cpp (batched queues of plans).  Internal.
+#pragma once
+
+#include <vector>
+
+#include "gr_internal.h"
+
+struct gr_plan {
+    gr_params prm;
+    GrGeom g;
+    std::vector<float> angles;  // copy of the caller's list (geom 0)
+    hipStream_t stream = nullptr;
+    // device tables
+    float *d_cose = nullptr, *d_sine = nullptr, *d_roi = nullptr, *d_wtbl = nullptr, *d_winv = nullptr;
+    float2 *d_filphase = nullptr, *d_tw_row = nullptr, *d_tw_grid = nullptr;
+    float *d_akey = nullptr;
+    int *d_aord = nullptr;
+    float4 *d_ang = nullptr;
+    unsigned *d_tiles = nullptr;
+    int ntiles = 0;
+    // workspace for max_pairs pairs: owned, or a region bound by a queue
+    int max_pairs = 1;
+    bool owns_workspace = true;
+    float2 *d_spec1 = nullptr, *d_spec2 = nullptr, *d_ht = nullptr, *d_z = nullptr;
+    size_t dev_bytes = 0;
+    // launch tuning (0 = automatic)
+    int cols_per_block = 0;
+    // host staging for gr_recon_host
+    float *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr;
+    // timing: event pairs recorded around launches, resolved in gr_get_timing
+    int timing = 0;
+    gr_timing t{};
+    struct Pending {
+        int kind;  // 0 rowfft, 1 grid, 2 ifft rows, 3 ifft cols
+        hipEvent_t e0, e1;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    GrDevTables tables() const {
+        GrDevTables t;
+        t.cose = d_cose;
+        t.sine = d_sine;
+        t.roi_offset = d_roi;
+        t.filphase = d_filphase;
+        t.wtbl = d_wtbl;
+        t.winv = d_winv;
+        t.tw_row = d_tw_row;
+        t.tw_grid = d_tw_grid;
+        t.akey = d_akey;
+        t.aord = d_aord;
+        t.ang = d_ang;
+        t.tiles = d_tiles;
+        t.ntiles = ntiles;
+        return t;
+    }
+};
+
+// Create a plan; with alloc_workspace false the plan gets no workspace and
+// must be bound to one (gr_plan_bind_workspace) before it runs.
+int gr_plan_create_impl(const gr_params *params, gr_plan **plan, bool alloc_workspace);
+// Workspace bytes one slice pair needs.
+size_t gr_plan_pair_bytes(const gr_plan *plan);
+// Point the plan's workspace at base (gr_plan_pair_bytes * max_pairs bytes,
+// 256-byte aligned), owned by the caller.
+void gr_plan_bind_workspace(gr_plan *plan, void *base, int max_pairs);
+int gr_set_error(int code, const char *msg);

@@ -125,7 +125,7 @@ static int cols_per_block(long M) {
 
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
                                long out_slice_stride, long out_row_stride, int n_slices, int pair0,
-                               int pairs, hipStream_t s) {
+                               int pairs, int cols, hipStream_t s) {
     ColsArgs A;
     A.z = z;
     A.out = out;
@@ -134,7 +134,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_slice_stride = out_slice_stride;
     A.out_row_stride = out_row_stride;
     A.M = (int)g.M;
-    A.cw = cols_per_block(g.M);
+    A.cw = cols > 0 && (long)cols * g.M <= 16384 ? cols : cols_per_block(g.M);
     A.n_slices = n_slices;
     A.pair0 = pair0;
     A.M02 = g.M02;
