@@ -13,6 +13,11 @@ dominant kernel from HIP events recorded around every launch in the timed
 region, accuracy of one pair against the reference's gridrec (its FP32
 restatement, and the reference build when present) and the FP64 oracle, and
 the oracle's CPU throughput on a bounded sample on this host.
+
+With --workload all (the default) the line also carries "batched": the uneven
+queue (4096 slices in datasets of 1-16 slices with widths 512..2560, 180-1800
+angles, centres n/2 + U[-5,5]) replayed as one HIP graph, against one plain
+plan per dataset on the same data, with bitwise equality of the images checked.
 """
 import argparse
 import json
@@ -43,6 +48,10 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--workload", default="all", choices=["all", "plain", "batched"],
+                    help="all: the plain headline line with the batched queue measured after it")
+    ap.add_argument("--queue-slices", type=int, default=4096)
+    ap.add_argument("--queue-streams", type=int, default=4)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 4 plain device copies of 1 GiB (reference for the HBM byte counters)")
     return ap.parse_args()
@@ -76,6 +85,126 @@ def kernel_bytes(kind, n_ang, n_det, half, M, out_n, pairs):
     return b * pairs
 
 
+QUEUE_WIDTHS = (512, 1024, 1536, 2048, 2560)
+
+
+def queue_datasets(gr, torch, seed, total):
+    """The uneven batch: datasets of 1-16 slices until `total` slices, each
+    with detector width uniform over QUEUE_WIDTHS, angle count uniform in
+    [180, 1800], centre n/2 + U[-5, 5], 20-ellipse phantoms with Poisson(1e4)
+    noise; half-padded on chip, output the n-wide window (round(M0/2) pixels)."""
+    rng = np.random.default_rng(seed)
+    sets, done, d = [], 0, 0
+    while done < total:
+        ns = int(min(rng.integers(1, 17), total - done))
+        n = int(rng.choice(QUEUE_WIDTHS))
+        na = int(rng.integers(180, 1801))
+        c = float(np.float32(n / 2 + rng.uniform(-5.0, 5.0)))
+        sino = torch.empty((ns, na, n), dtype=torch.float32, device="cuda")
+        gr.phantom_sinograms(sino, seed + 7919 * d, na, n, [c] * ns, 20, photons=1e4)
+        kw = dict(n_ang=na, n_det=n, center=c, pad="half")
+        M0 = (1 << (2 * n - 1).bit_length()) - 1  # M0 = pdim - 1 at sampl 1, pdim = 2^ceil(log2(2n))
+        cn = int(round(n * M0 / (2 * n)))
+        kw["crop"] = ((M0 - cn + 1) // 2, cn)
+        out = torch.empty((ns, cn, cn), dtype=torch.float32, device="cuda")
+        sets.append((kw, sino, out))
+        done += ns
+        d += 1
+    torch.cuda.synchronize()
+    return sets
+
+
+def bit_checksums(torch, sets):
+    """Per dataset: (sum of the images' bit patterns as int64, float64 sum)."""
+    return [(int(o.view(torch.int32).sum(dtype=torch.int64)), float(o.sum(dtype=torch.float64))) for _, _, o in sets]
+
+
+def bench_batched(a, gr, O, torch):
+    """The batched uneven queue against per-dataset plain plans on the same
+    datasets: execution time of both (HIP events), queue build time, bitwise
+    equality of the images (checksums), one pair against the oracle."""
+    t0 = time.time()
+    sets = queue_datasets(gr, torch, a.seed + 1, a.queue_slices)
+    gen_s = time.time() - t0
+    n_slices = sum(x[1].shape[0] for x in sets)
+    widths = {w: sum(x[1].shape[0] for x in sets if x[0]["n_det"] == w) for w in QUEUE_WIDTHS}
+
+    # per-dataset plain launches: one plan per dataset (built outside the
+    # timed region), recon_device on one stream
+    stream = torch.cuda.Stream()  # a stream of its own: events and launches on one queue
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    plain_ms, setup_s = 0.0, 0.0
+    for kw, sino, out in sets:
+        ts = time.perf_counter()
+        p = gr.Plan(**kw)
+        torch.cuda.synchronize()
+        setup_s += time.perf_counter() - ts
+        e0.record(stream)
+        p.recon(sino, out, stream=stream)
+        e1.record(stream)
+        e1.synchronize()
+        plain_ms += e0.elapsed_time(e1)
+        p.close()
+    ref_sums = bit_checksums(torch, sets)
+    for _, _, o in sets:
+        o.zero_()
+    torch.cuda.synchronize()
+
+    ts = time.perf_counter()
+    q = gr.Queue(sets, n_streams=a.queue_streams, autotune=True)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - ts
+    for _ in range(a.warmup):
+        q.run(stream)
+    torch.cuda.synchronize()
+    steps = max(1, a.steps)
+    e0.record(stream)
+    for _ in range(steps):
+        q.run(stream)
+    e1.record(stream)
+    e1.synchronize()
+    queue_ms = e0.elapsed_time(e1) / steps
+    same = bit_checksums(torch, sets) == ref_sums
+
+    # one pair of a small dataset against the oracle restatement and FP64
+    acc = None
+    if not a.no_accuracy:
+        for kw, sino, out in sets:
+            if sino.shape[0] >= 2 and kw["n_det"] <= 1024:
+                g = O.Geometry(n_ang=kw["n_ang"], n_det=kw["n_det"], center=kw["center"]).padded()
+                s = sino[0:2].cpu().numpy()
+                o = out[0:2].cpu().numpy()
+                p1, p2 = O.pad_half(s[0]), O.pad_half(s[1])
+                off, cn = kw["crop"]
+                c = slice(off, off + cn)
+                r1, r2 = O.recon_f32(g, p1, p2)
+                d1, d2 = O.recon_f64(g, p1, p2)
+                acc = {"dataset": {"n_det": kw["n_det"], "n_ang": kw["n_ang"], "center": kw["center"]},
+                       "rel_l2_vs_reference_restatement": float(f"{max(O.rel_l2(o[0], r1[c, c]), O.rel_l2(o[1], r2[c, c])):.3e}"),
+                       "rel_l2_vs_fp64": float(f"{max(O.rel_l2(o[0], d1[c, c]), O.rel_l2(o[1], d2[c, c])):.3e}")}
+                break
+    info = q.info
+    settings = {}
+    for d, (kw, _, _) in enumerate(sets):
+        st = q.settings(d)
+        settings.setdefault(f"{kw['n_det']}", (st["max_pairs"], st["cols_per_block"]))
+    q.close()
+    return {
+        "workload": f"batched uneven queue: {n_slices} slices in {len(sets)} datasets of 1-16 slices, "
+                    f"width in {list(QUEUE_WIDTHS)}, 180-1800 angles, centre n/2+U[-5,5], Poisson(1e4)",
+        "value": round(n_slices / (queue_ms * 1e-3), 2), "unit": "slices/s",
+        "per_dataset_plain_slices_s": round(n_slices / (plain_ms * 1e-3), 2),
+        "speedup_vs_per_dataset_plain": round(plain_ms / queue_ms, 3),
+        "queue_ms_per_batch": round(queue_ms, 2), "per_dataset_plain_ms": round(plain_ms, 2),
+        "per_dataset_plan_setup_s": round(setup_s, 2), "queue_build_s": round(build_s, 2),
+        "bitwise_equal_to_per_dataset_plain": bool(same),
+        "slices_by_width": widths, "streams": info["n_streams"], "size_classes": info["size_classes"],
+        "tuned_candidates": info["tuned_candidates"], "graph_nodes": info["graph_nodes"],
+        "tuned_pairs_and_cols_by_width": settings,
+        "workspace_GB": round(info["workspace_bytes"] / 1e9, 2), "gen_s": round(gen_s, 2), "accuracy": acc,
+    }
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -91,6 +220,15 @@ def main():
     from projects_amd import gridrec as gr
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.workload == "batched":
+        b = bench_batched(a, gr, O, torch)
+        line = {"metric": METRIC, "value": b.pop("value"), "unit": "slices/s", "n_gpus": 1, "steps": a.steps,
+                "warmup": a.warmup, "ms_per_step": b["queue_ms_per_batch"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic: seeded 20-ellipse phantoms, analytic Radon, Poisson(1e4)+log",
+                "config": {"workload": b.pop("workload")}, "batched": b}
+        print(json.dumps(line), flush=True)
+        return
     dev = torch.device("cuda:0")
     ns, nang, ndet = a.slices, a.n_ang, a.n_det
     rng = np.random.default_rng(a.seed)
@@ -110,7 +248,7 @@ def main():
     inf = plan.info
     out_n = plan.out_n
     out = torch.empty((ns, out_n, out_n), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # a stream of its own: events and launches on one queue
 
     for _ in range(a.warmup):
         plan.recon(sino, out)
@@ -144,6 +282,16 @@ def main():
     d = per_kernel[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(d["gbs"] / HBM_PEAK_GBS, 5), "traffic": None}
+    # measured HBM bytes per launch of this kernel at this workload, from the
+    # FETCH_SIZE / WRITE_SIZE counter runs kept beside the profile summaries
+    tf = os.path.join(ROOT, "profiles", "gridrec_plain_1500x2048_hbm_traffic.json")
+    if os.path.exists(tf) and (nang, ndet, inf["max_pairs"], a.accum) == (1500, 2048, 8, "compensated"):
+        with open(tf) as f:
+            k = json.load(f)["kernels"].get(dom)
+        if k and k["pairs_per_launch"] == 8:
+            roofline["traffic"] = k["total_bytes"]
+            roofline["traffic_read_write"] = [k["read_bytes"], k["write_bytes"]]
+            roofline["traffic_source"] = "profiles/gridrec_plain_1500x2048_hbm_traffic.md"
     # whole-pipeline algorithmic bytes per pair (8 n_ang n_det + 16 n_ang pdim + 32 M^2 + 8 n_out^2)
     b_pair = 8 * nang * ndet + 16 * nang * inf["pdim"] + 32 * M * M + 8 * out_n * out_n
     pipeline_gbs = b_pair * (slices_s / 2) / 1e9
@@ -196,8 +344,12 @@ def main():
                                   "kind": "port",
                                   "sample": f"{2 * pairs} slices ({pairs} pairs) of the same workload, FP32 "
                                             f"restatement of GridrecCPP, {threads} threads, {cdt:.1f} s"}
-    print(json.dumps(result), flush=True)
     plan.close()
+    if a.workload == "all":
+        del sino, out
+        torch.cuda.empty_cache()
+        result["batched"] = bench_batched(a, gr, O, torch)
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -2,7 +2,8 @@
  * gridrec_hip.h -- C-ABI of the MI355X gridrec library (libgridrec_hip.so).
  *
  * Plain C types only: pointers, sizes, status codes.  A HIP stream is passed
- * as an opaque `void *` (a hipStream_t; NULL = the plan's own stream).
+ * as an opaque `void *` (a hipStream_t; NULL = the null stream, which orders
+ * the work with the device's other blocking streams).
  *
  * Each entry point names the reference interface it replaces
  * (gridrec/GridrecCPP/Gridrec5.2 unless stated):
@@ -159,7 +160,8 @@ int gr_recon_device(gr_plan *plan, const float *sino, long sino_slice_stride,
 /* recon_free. */
 int gr_plan_destroy(gr_plan *plan);
 
-/* Per-kernel HIP-event timing of gr_recon_device (adds a sync per launch). */
+/* Per-kernel HIP-event timing of gr_recon_device: an event pair around each
+ * launch, resolved (waited for) by gr_get_timing. */
 int gr_set_timing(gr_plan *plan, int enable);
 int gr_get_timing(gr_plan *plan, gr_timing *out, int reset);
 
@@ -173,6 +175,49 @@ int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slic
                          unsigned long long seed, int n_ang, int n_det,
                          const float *centers, int n_ell, float photons, int density,
                          float gauss_rel, void *stream);
+
+/* Batched reconstruction of many datasets of uneven geometry, all resident in
+ * device memory -- the dataset loop of the reference's drivers
+ * (Multitomo_unit.txt:823-856 runs recon_init/do_recon/recon_free once per
+ * dataset; tomo_recon.cpp:3700-3863 once per slice batch) made one launch.
+ * Each dataset has its own gr_params and buffers (arguments as in
+ * gr_recon_device).  gr_queue_create builds one plan per dataset, autotunes
+ * slice pairs per launch and the column-FFT width per size class (when
+ * autotune != 0), spreads the datasets over n_streams streams (1..8) and
+ * captures all their launches into one HIP graph; it runs the batch while
+ * tuning, so the output buffers are written.  gr_queue_run replays the graph
+ * on `stream`, asynchronously; the images
+ * are bitwise those of gr_recon_device on each dataset.  The buffers named at
+ * create are the ones every run reads and writes. */
+typedef struct gr_dataset {
+    gr_params params;
+    const float *sino;
+    long sino_slice_stride, sino_row_stride;
+    float *out;
+    long out_slice_stride, out_row_stride;
+    int n_slices;
+} gr_dataset;
+
+typedef struct gr_queue_info {
+    int n_datasets, n_streams, size_classes, tuned_candidates;
+    long slices;            /* total slices of the batch */
+    long graph_nodes;
+    size_t workspace_bytes; /* shared per-stream workspaces */
+} gr_queue_info;
+
+typedef struct gr_queue gr_queue;
+
+int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int n_streams,
+                    int autotune, gr_queue **queue);
+int gr_queue_run(gr_queue *queue, void *stream);
+/* The same launches without the graph, every dataset in turn on one stream. */
+int gr_queue_run_sequential(gr_queue *queue, void *stream);
+int gr_queue_get_info(const gr_queue *queue, gr_queue_info *info);
+/* Tuned settings of dataset d: pairs per launch, column-FFT width (0 =
+ * automatic), stream index.  Any pointer may be NULL. */
+int gr_queue_dataset_settings(const gr_queue *queue, int d, int *max_pairs, int *cols_per_block,
+                              int *stream);
+int gr_queue_destroy(gr_queue *queue);
 
 const char *gr_last_error(void);
 const char *gr_version(void);

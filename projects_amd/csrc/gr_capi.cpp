@@ -13,8 +13,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "gr_plan.h"
@@ -60,12 +63,71 @@ int dev_alloc(gr_plan *pl, T **dst, size_t n) {
     return GR_OK;
 }
 
+// Tables that depend only on a transform size -- twiddles and the gridding
+// tile order -- are built once per (device, kind, size) and shared by every
+// plan that needs them, reference-counted: a batch of hundreds of datasets
+// holds one copy per size instead of one per dataset.
+struct SharedTable {
+    void *ptr;
+    int refs;
+};
+std::mutex g_shared_mu;
+std::map<std::tuple<int, int, long>, SharedTable> g_shared;
+enum { kTwPlus = 0, kTwMinus = 1, kTiles = 2 };
+
+template <typename T, typename Build>
+int shared_table(int device, int kind, long n, Build build, T **out, int *count) {
+    std::lock_guard<std::mutex> lock(g_shared_mu);
+    auto key = std::make_tuple(device, kind, n);
+    auto it = g_shared.find(key);
+    if (it == g_shared.end()) {
+        std::vector<T> h = build();
+        void *d = nullptr;
+        hipError_t e = hipMalloc(&d, sizeof(T) * (h.empty() ? 1 : h.size()));
+        if (e == hipSuccess && !h.empty()) e = hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (d) (void)hipFree(d);
+            return fail(GR_ENOMEM, std::string("shared table: ") + hipGetErrorString(e));
+        }
+        it = g_shared.emplace(key, SharedTable{d, 0}).first;
+        if (count) *count = (int)h.size();
+    } else if (count) {
+        *count = -1;  // caller already knows the size
+    }
+    it->second.refs++;
+    *out = (T *)it->second.ptr;
+    return GR_OK;
+}
+
+void shared_release(const void *ptr) {
+    if (!ptr) return;
+    std::lock_guard<std::mutex> lock(g_shared_mu);
+    for (auto it = g_shared.begin(); it != g_shared.end(); ++it) {
+        if (it->second.ptr != ptr) continue;
+        if (--it->second.refs == 0) {
+            (void)hipFree(it->second.ptr);
+            g_shared.erase(it);
+        }
+        return;
+    }
+}
+
+int plan_stream(gr_plan *pl, hipStream_t *s) {
+    if (!pl->stream && hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(GR_EHIP, "stream creation failed");
+    *s = pl->stream;
+    return GR_OK;
+}
+
 void free_plan(gr_plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase,
-                    pl->d_tw_row, pl->d_tw_grid, pl->d_akey, pl->d_aord, pl->d_ang, pl->d_tiles, pl->d_in, pl->d_out};
+                    pl->d_akey, pl->d_aord, pl->d_ang, pl->d_in, pl->d_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    shared_release(pl->d_tw_row);
+    shared_release(pl->d_tw_grid);
+    shared_release(pl->d_tiles);
     if (pl->owns_workspace) {
         void *ws[] = {pl->d_spec1, pl->d_spec2, pl->d_ht, pl->d_z};
         for (void *p : ws)
@@ -235,7 +297,6 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         }                      \
     } while (0)
     CK(hipSetDevice(params->device) == hipSuccess ? 0 : fail(GR_EHIP, "hipSetDevice failed"));
-    CK(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess ? 0 : fail(GR_EHIP, "stream"));
 
     // reference tables (host, bit-identical to recon_init)
     std::vector<float> sine(n_ang), cose(n_ang), roi(n_ang), filphase(2 * g.half), wtbl(params->ltbl + 1),
@@ -248,17 +309,19 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
     CK(dev_upload(pl, &pl->d_wtbl, wtbl.data(), wtbl.size()));
     CK(dev_upload(pl, &pl->d_winv, winv.data(), winv.size()));
     if (params->tables == GR_TABLES_FP64) {
-        hipError_t e = gr_launch_tables_fp64(g, &pl->prm, pl->d_filphase, pl->d_wtbl, pl->d_winv, pl->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(pl->stream);
+        hipStream_t ts;
+        CK(plan_stream(pl, &ts));
+        hipError_t e = gr_launch_tables_fp64(g, &pl->prm, pl->d_filphase, pl->d_wtbl, pl->d_winv, ts);
+        if (e == hipSuccess) e = hipStreamSynchronize(ts);
         if (e != hipSuccess) {
             free_plan(pl);
             return hip_fail(e, "FP64 tables");
         }
     }
-    // twiddles
-    std::vector<float2> tr = twiddles(g.pdim, +1), tg = twiddles(g.M, -1);
-    CK(dev_upload(pl, &pl->d_tw_row, tr.data(), tr.size()));
-    CK(dev_upload(pl, &pl->d_tw_grid, tg.data(), tg.size()));
+    // twiddles (shared per size)
+    CK(shared_table(params->device, kTwPlus, g.pdim, [&] { return twiddles(g.pdim, +1); }, &pl->d_tw_row, nullptr));
+    CK(shared_table(params->device, kTwMinus, g.M, [&] { return twiddles(g.M, -1); }, &pl->d_tw_grid, nullptr));
+    pl->dev_bytes += sizeof(float2) * (g.pdim + g.M);
     // angle keys: direction of each sample line, atan2(SINE, COSE) mod pi, sorted
     {
         std::vector<double> key(n_ang);
@@ -284,36 +347,47 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         }
         CK(dev_upload(pl, &pl->d_ang, ang.data(), n_ang));
     }
-    // tiles of the gridding kernel over the M x M grid, nearest to the centre first
+    // tiles of the gridding kernel over the M x M grid, nearest to the centre
+    // first (shared per size)
     {
         const int ts = gr_grid_tile_size();
         const int nt = (int)((g.M + ts - 1) / ts);
-        const double c = (double)(g.M >> 1), h = 0.5 * (ts - 1);
-        std::vector<unsigned> tiles;
-        std::vector<double> dist;
-        tiles.reserve((size_t)nt * nt);
-        for (int ty = 0; ty < nt; ty++)
-            for (int tx = 0; tx < nt; tx++) tiles.push_back((unsigned)tx | ((unsigned)ty << 16));
-        dist.resize(tiles.size());
-        for (size_t i = 0; i < tiles.size(); i++) {
-            const double x = (double)(tiles[i] & 0xffff) * ts + h - c, y = (double)(tiles[i] >> 16) * ts + h - c;
-            dist[i] = x * x + y * y;
-        }
-        std::vector<size_t> idx(tiles.size());
-        std::iota(idx.begin(), idx.end(), 0);
-        std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return dist[x] < dist[y]; });
-        std::vector<unsigned> sorted(tiles.size());
-        for (size_t i = 0; i < idx.size(); i++) sorted[i] = tiles[idx[i]];
-        pl->ntiles = (int)sorted.size();
-        CK(dev_upload(pl, &pl->d_tiles, sorted.data(), sorted.size()));
+        if ((long)nt * nt >= (1L << 24)) CK(fail(GR_EINVAL, "grid too large for the tile list"));
+        auto build = [&] {
+            // key: squared distance of the tile centre from the grid centre, in
+            // units of 1/4 cell (exact integers), then the row-major index
+            const long c2 = 2 * (g.M >> 1), h2 = ts - 1;
+            std::vector<unsigned long long> key((size_t)nt * nt);
+            for (int ty = 0; ty < nt; ty++)
+                for (int tx = 0; tx < nt; tx++) {
+                    const long x = 2L * tx * ts + h2 - c2, y = 2L * ty * ts + h2 - c2;
+                    const size_t i = (size_t)ty * nt + tx;
+                    key[i] = ((unsigned long long)(x * x + y * y) << 24) | i;
+                }
+            std::sort(key.begin(), key.end());
+            std::vector<unsigned> sorted(key.size());
+            for (size_t i = 0; i < key.size(); i++) {
+                const unsigned idx = (unsigned)(key[i] & 0xffffff);
+                sorted[i] = (idx % (unsigned)nt) | ((idx / (unsigned)nt) << 16);
+            }
+            return sorted;
+        };
+        CK(shared_table(params->device, kTiles, g.M, build, &pl->d_tiles, nullptr));
+        pl->ntiles = nt * nt;
+        pl->dev_bytes += sizeof(unsigned) * (size_t)nt * nt;
     }
     // workspace
     const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
     const size_t per_pair = sizeof(float2) * (2 * spec_pp + ht_pp + z_pp);
     int mp = params->max_pairs;
     if (mp <= 0) {
-        const size_t budget = (size_t)2 << 30;
-        mp = (int)std::max<size_t>(1, std::min<size_t>(16, budget / per_pair));
+        // 8 pairs per launch -- the widest gridding kernel, which reads each
+        // sample's geometry once for all 8 -- when a quarter of the free
+        // device memory (at most 8 GiB) holds them, fewer otherwise
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = (size_t)8 << 30;
+        const size_t budget = std::min<size_t>(free_b / 4, (size_t)8 << 30);
+        mp = (int)std::max<size_t>(1, std::min<size_t>(8, budget / per_pair));
     }
     pl->max_pairs = mp;
     if (alloc_workspace) {
@@ -373,8 +447,8 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
         return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
     if (out_row_stride < g.out_n || out_slice_stride < out_row_stride * g.out_n)
         return fail(GR_EINVAL, "image strides smaller than the window they hold");
-    hipStream_t s = stream ? (hipStream_t)stream : pl->stream;
     GR_HIP(hipSetDevice(pl->prm.device));
+    hipStream_t s = (hipStream_t)stream;  // NULL: the null stream
     const GrDevTables t = pl->tables();
     const int n_pairs = (n_slices + 1) / 2;
     for (int p0 = 0; p0 < n_pairs; p0 += pl->max_pairs) {
@@ -427,11 +501,14 @@ int gr_recon_host(gr_plan *pl, const float *const *G1, const float *const *G2, f
         memcpy(pl->h_in + (size_t)n * nd, G1[n], sizeof(float) * nd);
         memcpy(pl->h_in + (size_t)(n_ang + n) * nd, G2[n], sizeof(float) * nd);
     }
-    GR_HIP(hipMemcpyAsync(pl->d_in, pl->h_in, sizeof(float) * in_n, hipMemcpyHostToDevice, pl->stream));
-    int rc = gr_recon_device(pl, pl->d_in, (long)n_ang * nd, nd, pl->d_out, on * on, on, 2, pl->stream);
+    hipStream_t st;
+    int rc = plan_stream(pl, &st);
     if (rc) return rc;
-    GR_HIP(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(float) * out_n, hipMemcpyDeviceToHost, pl->stream));
-    GR_HIP(hipStreamSynchronize(pl->stream));
+    GR_HIP(hipMemcpyAsync(pl->d_in, pl->h_in, sizeof(float) * in_n, hipMemcpyHostToDevice, st));
+    rc = gr_recon_device(pl, pl->d_in, (long)n_ang * nd, nd, pl->d_out, on * on, on, 2, st);
+    if (rc) return rc;
+    GR_HIP(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(float) * out_n, hipMemcpyDeviceToHost, st));
+    GR_HIP(hipStreamSynchronize(st));
     for (long j = 0; j < on; j++) {
         memcpy(S1[j], pl->h_out + j * on, sizeof(float) * on);
         memcpy(S2[j], pl->h_out + (on + j) * on, sizeof(float) * on);

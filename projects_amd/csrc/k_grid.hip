@@ -51,7 +51,7 @@ struct GridArgs {
     const float *akey;   // sorted keys
     const unsigned *tiles;
     const float *wtbl;
-    int ntiles, n_ang, half, lds_wtbl, pair_base;
+    int ntiles, n_ang, half, lds_wtbl, pair_base, pair_end;
     GrGatherGeom g;
     float L2m, dmax, rmax;
 };
@@ -67,15 +67,16 @@ __device__ __forceinline__ void accumulate(float4 &acc, float xr, float xi) {
     }
 }
 
-// Stage samples c0 .. c0+nj-1 of angle n for the P pairs, then add the taps
-// of samples [j0, j1] into cell (qa, qb).
+// Stage samples c0 .. c0+nj-1 of angle n for the P pairs (zeros for the
+// nvalid..P-1 slots of a partial group), then add the taps of samples
+// [j0, j1] into cell (qa, qb).
 template <int ACCUM, int P>
-__device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, long row_off, int c0, int nj, int j0,
+__device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int nvalid, long row_off, int c0, int nj, int j0,
                                       int j1, float C, float S, float qa, float qb, const GrGatherGeom &g,
                                       const float *wtbl, float2 (*stage)[64], int lane, float4 (&acc)[P]) {
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (lane < nj) stage[p][lane] = spec[p * pair_stride + row_off + c0 + lane];
+        if (lane < nj) stage[p][lane] = p < nvalid ? spec[p * pair_stride + row_off + c0 + lane] : make_float2(0.0f, 0.0f);
     __builtin_amdgcn_wave_barrier();
     for (int j = j0; j <= j1; j++) {
         float w;
@@ -102,6 +103,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     if (ti >= A.ntiles) return;
     float2(*stage)[64] = reinterpret_cast<float2(*)[64]>(smem + A.lds_wtbl) + wave * P;
     const int pair0 = A.pair_base + blockIdx.y * P;
+    const int nvalid = min(P, A.pair_end - pair0);
     const unsigned tt = A.tiles[ti];
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
@@ -140,10 +142,10 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
                 }
                 const long row_off = (long)A.aord[k] * A.half;
                 for (int c0 = td0; c0 <= td1; c0 += 64)
-                    chunk<ACCUM, P>(base1, pair_stride, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0),
+                    chunk<ACCUM, P>(base1, pair_stride, nvalid, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0),
                                     min(min(jd1, td1), c0 + 63), cs.x, cs.y, pa, pb, A.g, s_wtbl, stage, lane, acc);
                 for (int c0 = tm0; c0 <= tm1; c0 += 64)
-                    chunk<ACCUM, P>(base2, pair_stride, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0),
+                    chunk<ACCUM, P>(base2, pair_stride, nvalid, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0),
                                     min(min(jm1, tm1), c0 + 63), cs.x, cs.y, ma, mb, A.g, s_wtbl, stage, lane, acc);
             }
         }
@@ -151,6 +153,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     if (a < M && b < M) {
 #pragma unroll
         for (int p = 0; p < P; p++) {
+            if (p >= nvalid) break;
             const float4 s = acc[p];
             const float2 h = ACCUM == GR_ACCUM_COMPENSATED ? make_float2(s.x - s.y, s.z - s.w) : make_float2(s.x, s.z);
             A.ht[((long)(pair0 + p) * M + b) * M + a] = h;
@@ -169,29 +172,20 @@ hipError_t launch(GridArgs A, int groups, int ntiles, size_t lds_base, hipStream
 
 template <int ACCUM>
 hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, hipStream_t s) {
-    // pairs in groups of 8, then 4, 2, 1 for the remainder
+    // groups of 8 pairs; a remainder of 5-7 rides in one more (partial) group
+    // of 8, one of 3-4 in a group of 4, 2 and 1 in their own widths: a tap's
+    // geometry costs the same for any width, so one partial wide group beats
+    // several narrow launches
+    const int full = pairs / 8, rem = pairs % 8;
     hipError_t e = hipSuccess;
-    int done = 0;
-    if (pairs - done >= 8) {
-        A.pair_base = done;
-        e = launch<ACCUM, 8>(A, (pairs - done) / 8, ntiles, lds_base, s);
-        done += (pairs - done) / 8 * 8;
-    }
-    if (e == hipSuccess && pairs - done >= 4) {
-        A.pair_base = done;
-        e = launch<ACCUM, 4>(A, 1, ntiles, lds_base, s);
-        done += 4;
-    }
-    if (e == hipSuccess && pairs - done >= 2) {
-        A.pair_base = done;
-        e = launch<ACCUM, 2>(A, 1, ntiles, lds_base, s);
-        done += 2;
-    }
-    if (e == hipSuccess && pairs - done >= 1) {
-        A.pair_base = done;
-        e = launch<ACCUM, 1>(A, 1, ntiles, lds_base, s);
-    }
-    return e;
+    A.pair_end = pairs;
+    A.pair_base = 0;
+    if (full + (rem >= 5) > 0) e = launch<ACCUM, 8>(A, full + (rem >= 5), ntiles, lds_base, s);
+    if (e != hipSuccess || rem == 0 || rem >= 5) return e;
+    A.pair_base = full * 8;
+    if (rem >= 3) return launch<ACCUM, 4>(A, 1, ntiles, lds_base, s);
+    if (rem == 2) return launch<ACCUM, 2>(A, 1, ntiles, lds_base, s);
+    return launch<ACCUM, 1>(A, 1, ntiles, lds_base, s);
 }
 
 }  // namespace

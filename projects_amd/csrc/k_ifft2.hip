@@ -117,10 +117,14 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
     return hipGetLastError();
 }
 
-// Columns per workgroup: as many M-point columns as fit in 128 KiB of LDS.
-static int cols_per_block(long M) {
+// Columns per workgroup: as many M-point columns as fit in 128 KiB of LDS,
+// halved while the launch would hold fewer than 8 workgroups per CU (small
+// windows or few pairs leave most of the 256 CUs idle otherwise).
+static int cols_per_block(long M, long out_n, int pairs) {
     long cw = 16384 / M;
-    return (int)(cw < 1 ? 1 : (cw > 16 ? 16 : cw));
+    cw = cw < 1 ? 1 : (cw > 16 ? 16 : cw);
+    while (cw > 1 && ((out_n + cw - 1) / cw) * pairs < 2048) cw >>= 1;
+    return (int)cw;
 }
 
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
@@ -134,7 +138,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_slice_stride = out_slice_stride;
     A.out_row_stride = out_row_stride;
     A.M = (int)g.M;
-    A.cw = cols > 0 && (long)cols * g.M <= 16384 ? cols : cols_per_block(g.M);
+    A.cw = cols > 0 && (long)cols * g.M <= 16384 ? cols : cols_per_block(g.M, g.out_n, pairs);
     A.n_slices = n_slices;
     A.pair0 = pair0;
     A.M02 = g.M02;

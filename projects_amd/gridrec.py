@@ -56,6 +56,20 @@ class GrTiming(C.Structure):
     ]
 
 
+class GrDataset(C.Structure):
+    _fields_ = [
+        ("params", GrParams), ("sino", C.c_void_p), ("sino_slice_stride", C.c_long), ("sino_row_stride", C.c_long),
+        ("out", C.c_void_p), ("out_slice_stride", C.c_long), ("out_row_stride", C.c_long), ("n_slices", C.c_int),
+    ]
+
+
+class GrQueueInfo(C.Structure):
+    _fields_ = [
+        ("n_datasets", C.c_int), ("n_streams", C.c_int), ("size_classes", C.c_int), ("tuned_candidates", C.c_int),
+        ("slices", C.c_long), ("graph_nodes", C.c_long), ("workspace_bytes", C.c_size_t),
+    ]
+
+
 # every function of include/gridrec_hip.h with its ctypes signature
 _SIGS = {
     "gr_get_pswf": (C.c_int, [C.c_float, C.POINTER(GrPswf)]),
@@ -71,6 +85,14 @@ _SIGS = {
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
     "gr_phantom_sinograms": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_int,
                                        C.POINTER(C.c_float), C.c_int, C.c_float, C.c_int, C.c_float, C.c_void_p]),
+    "gr_queue_create": (C.c_int, [C.POINTER(GrDataset), C.c_int, C.c_int, C.c_int, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    "gr_queue_run": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gr_queue_run_sequential": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gr_queue_get_info": (C.c_int, [C.c_void_p, C.POINTER(GrQueueInfo)]),
+    "gr_queue_dataset_settings": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int)]),
+    "gr_queue_destroy": (C.c_int, [C.c_void_p]),
     "gr_last_error": (C.c_char_p, []),
     "gr_version": (C.c_char_p, []),
 }
@@ -127,6 +149,23 @@ def _stream_handle(stream) -> Optional[int]:
     return getattr(stream, "cuda_stream", stream)
 
 
+def make_params(n_ang: int, n_det: int, center: float, geom: int = 1, angles: Optional[Sequence[float]] = None,
+                filter: str = "shlo", C_: float = 6.0, sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0,
+                X0: float = 0.0, Y0: float = 0.0, ltbl: int = 512, pad: str = "none",
+                crop: Optional[Tuple[int, int]] = None, accum: str = "compensated", tables: str = "reference",
+                device: int = 0, max_pairs: int = 0):
+    """gr_params of one geometry, and the angle array it points to (keep it alive)."""
+    ang = None
+    ang_ptr = None
+    if angles is not None:
+        ang = np.ascontiguousarray(np.asarray(angles, np.float32))
+        ang_ptr = ang.ctypes.data_as(C.POINTER(C.c_float))
+    crop_off, crop_n = crop if crop is not None else (0, 0)
+    prm = GrParams(n_ang, n_det, geom, ang_ptr, center, C_, sampl, MaxPixSiz, R, X0, Y0, filter.encode()[:15], ltbl,
+                   PAD[pad], crop_off, crop_n, ACCUM[accum], TABLES[tables], device, max_pairs, None, None)
+    return prm, ang
+
+
 class Plan:
     """A gridrec set-up for one geometry (recon_init) on one GPU."""
 
@@ -135,15 +174,8 @@ class Plan:
                  sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0, X0: float = 0.0, Y0: float = 0.0,
                  ltbl: int = 512, pad: str = "none", crop: Optional[Tuple[int, int]] = None,
                  accum: str = "compensated", tables: str = "reference", device: int = 0, max_pairs: int = 0):
-        self._angles = None
-        ang_ptr = None
-        if angles is not None:
-            self._angles = np.ascontiguousarray(np.asarray(angles, np.float32))
-            ang_ptr = self._angles.ctypes.data_as(C.POINTER(C.c_float))
-        crop_off, crop_n = crop if crop is not None else (0, 0)
-        self.params = GrParams(n_ang, n_det, geom, ang_ptr, center, C_, sampl, MaxPixSiz, R, X0, Y0,
-                               filter.encode()[:15], ltbl, PAD[pad], crop_off, crop_n, ACCUM[accum],
-                               TABLES[tables], device, max_pairs, None, None)
+        self.params, self._angles = make_params(n_ang, n_det, center, geom, angles, filter, C_, sampl, MaxPixSiz, R,
+                                                X0, Y0, ltbl, pad, crop, accum, tables, device, max_pairs)
         h = C.c_void_p()
         _check(lib().gr_plan_create(C.byref(self.params), C.byref(h)), "gr_plan_create")
         self._h = h
@@ -221,6 +253,59 @@ class Plan:
         t = GrTiming()
         _check(lib().gr_get_timing(self._h, C.byref(t), int(reset)), "gr_get_timing")
         return {f: getattr(t, f) for f, _ in GrTiming._fields_}
+
+
+class Queue:
+    """A batch of datasets of uneven geometry reconstructed as one HIP graph
+    (gr_queue_*).  datasets: list of (params_kwargs, sino, out) where sino is a
+    cuda float32 [n_slices, n_ang, >=n_det] tensor, out a cuda float32
+    [n_slices, >=out_n, >=out_n] tensor and params_kwargs the keyword
+    arguments of make_params.  Creating the queue runs the batch once."""
+
+    def __init__(self, datasets, device: int = 0, n_streams: int = 4, autotune: bool = True):
+        import torch
+
+        arr = (GrDataset * len(datasets))()
+        self._keep = []
+        for i, (kw, sino, out) in enumerate(datasets):
+            prm, ang = make_params(device=device, **kw)
+            self._keep += [ang, sino, out]
+            assert sino.is_cuda and sino.dtype == torch.float32 and sino.dim() == 3 and sino.stride(2) == 1
+            assert out.is_cuda and out.dtype == torch.float32 and out.dim() == 3 and out.stride(2) == 1
+            assert out.shape[0] == sino.shape[0]
+            arr[i] = GrDataset(prm, sino.data_ptr(), sino.stride(0), sino.stride(1), out.data_ptr(), out.stride(0),
+                               out.stride(1), sino.shape[0])
+        h = C.c_void_p()
+        _check(lib().gr_queue_create(arr, len(datasets), device, n_streams, int(autotune), C.byref(h)),
+               "gr_queue_create")
+        self._h = h
+        inf = GrQueueInfo()
+        _check(lib().gr_queue_get_info(self._h, C.byref(inf)), "gr_queue_get_info")
+        self.info = {f: getattr(inf, f) for f, _ in GrQueueInfo._fields_}
+
+    def run(self, stream=None):
+        _check(lib().gr_queue_run(self._h, C.c_void_p(_stream_handle(stream))), "gr_queue_run")
+
+    def run_sequential(self, stream=None):
+        _check(lib().gr_queue_run_sequential(self._h, C.c_void_p(_stream_handle(stream))),
+               "gr_queue_run_sequential")
+
+    def settings(self, d: int) -> dict:
+        mp, cw, st = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().gr_queue_dataset_settings(self._h, d, C.byref(mp), C.byref(cw), C.byref(st)),
+               "gr_queue_dataset_settings")
+        return dict(max_pairs=mp.value, cols_per_block=cw.value, stream=st.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().gr_queue_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def phantom_sinograms(out, seed: int, n_ang: int, n_det: int, centers, n_ell: int, photons: float = 0.0,

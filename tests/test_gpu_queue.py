@@ -1,0 +1,121 @@
+"""GPU tests of the batched uneven queue (gr_queue_*), through the C-ABI.
+
+The queue's images must be bitwise those of a plain per-dataset plan
+(gr_recon_device with default settings): the tuned knobs -- slice pairs per
+launch and the column-FFT width -- change no sum order.  The graph is replayed
+several times with eager work of the program between replays, and every replay
+is checked against the eager result.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from projects_amd import gridrec as gr
+
+
+def uneven_batch(seed, n_sets, widths=(64, 96, 128, 192), slices=(1, 2, 3, 5, 8), angles=(24, 120)):
+    """Datasets of random width, angle count, centre and slice count, with
+    their sinograms generated on the device and their output buffers."""
+    rng = np.random.default_rng(seed)
+    sets = []
+    for d in range(n_sets):
+        n = int(rng.choice(widths))
+        na = int(rng.integers(*angles))
+        ns = int(rng.choice(slices))
+        c = n / 2 + float(rng.uniform(-3, 3))
+        sino = torch.empty((ns, na, n), dtype=torch.float32, device="cuda")
+        gr.phantom_sinograms(sino, seed=seed * 1000 + d, n_ang=na, n_det=n, centers=[c] * ns, n_ell=8,
+                             photons=1e4)
+        kw = dict(n_ang=na, n_det=n, center=c, filter=str(rng.choice(["shlo", "hann", "ramp"])), pad="half")
+        M0 = gr.Plan(**kw).info["M0"]
+        cn = int(round(n * M0 / (2 * n)))
+        kw["crop"] = ((M0 - cn + 1) // 2, cn)
+        out = torch.full((ns, cn, cn), float("nan"), dtype=torch.float32, device="cuda")
+        sets.append((kw, sino, out))
+    return sets
+
+
+def plain_results(sets):
+    res = []
+    for kw, sino, _ in sets:
+        p = gr.Plan(**kw)
+        res.append(p.recon(sino))
+        p.close()
+    torch.cuda.synchronize()
+    return [r.cpu() for r in res]
+
+
+def test_queue_matches_plain_plans_bitwise_and_replays():
+    sets = uneven_batch(11, 10)
+    ref = plain_results(sets)
+    q = gr.Queue(sets, n_streams=3, autotune=True)
+    torch.cuda.synchronize()
+    info = q.info
+    print("queue info", info, flush=True)
+    assert info["n_datasets"] == 10 and info["slices"] == sum(s[1].shape[0] for s in sets)
+    for d, (_, _, out) in enumerate(sets):  # create ran the batch once
+        assert torch.equal(out.cpu(), ref[d]), d
+    for rep in range(3):
+        for _, _, out in sets:
+            out.fill_(float("nan"))
+        scratch = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        q.run()
+        torch.cuda.synchronize()
+        assert torch.isfinite(scratch)
+        for d, (_, _, out) in enumerate(sets):
+            assert torch.equal(out.cpu(), ref[d]), (rep, d)
+    for _, _, out in sets:
+        out.fill_(float("nan"))
+    q.run_sequential()
+    torch.cuda.synchronize()
+    for d, (_, _, out) in enumerate(sets):
+        assert torch.equal(out.cpu(), ref[d]), ("sequential", d)
+    for d in range(10):
+        s = q.settings(d)
+        assert s["max_pairs"] in (1, 2, 4, 8) and 0 <= s["stream"] < 3
+    q.close()
+
+
+def test_queue_without_autotune_and_single_stream():
+    sets = uneven_batch(12, 5, slices=(2, 4))
+    ref = plain_results(sets)
+    q = gr.Queue(sets, n_streams=1, autotune=False)
+    for _, _, out in sets:
+        out.fill_(float("nan"))
+    q.run()
+    torch.cuda.synchronize()
+    for d, (_, _, out) in enumerate(sets):
+        assert torch.equal(out.cpu(), ref[d]), d
+    q.close()
+
+
+def test_queue_images_match_oracle():
+    """A queue dataset against the C oracle (FP32 restatement) on the padded rows."""
+    sets = uneven_batch(13, 4, slices=(2,))
+    q = gr.Queue(sets, n_streams=2)
+    q.run()
+    torch.cuda.synchronize()
+    for kw, sino, out in sets:
+        g = O.Geometry(n_ang=kw["n_ang"], n_det=kw["n_det"], center=kw["center"], filter=kw["filter"]).padded()
+        s = sino.cpu().numpy()
+        a, b = O.pad_half(s[0]), O.pad_half(s[1])
+        r1, r2 = O.recon_f32(g, a, b)
+        off, cn = kw["crop"]
+        o = out.cpu().numpy()
+        for img, r in ((o[0], r1), (o[1], r2)):
+            e = O.rel_l2(img, r[off:off + cn, off:off + cn])
+            assert e <= 1e-5, (kw, e)
+    q.close()
+
+
+def test_queue_rejects_bad_datasets():
+    sets = uneven_batch(14, 2, slices=(2,))
+    kw, sino, out = sets[1]
+    bad = dict(kw, n_ang=0)
+    with pytest.raises(gr.GridrecError):
+        gr.Queue([sets[0], (bad, sino, out)])
