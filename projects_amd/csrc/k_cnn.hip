@@ -1,0 +1,686 @@
+// k_cnn.hip -- the sparse-angle residual CNN: forward, backward and Adam.
+//
+// Network (no counterpart in the reference; it corrects gridrec's streaks on
+// sparse-angle scans): five 3x3 convolutions 1 -> 32 -> 32 -> 32 -> 32 -> 1,
+// ReLU after the first four, y = x/scale + conv5(a4) + b5.  BF16 weights and
+// activations, FP32 accumulation, MSE loss, Adam on FP32 master weights.
+//
+// Layouts: images [n][H][W] f32; activations and their gradients [n][H][W][32]
+// bf16 (channels last: one pixel is one 64-byte record); parameters one flat
+// f32 vector W1[32][9] b1[32] W2[32][32][9] b2 W3 b3 W4 b4 W5[32][9] b5[1]
+// (taps ky*3+kx, correlation, zero padding).
+//
+// The three 32 -> 32 layers run on MFMA (v_mfma_f32_16x16x32_bf16): a block
+// owns an 8 x 32 pixel tile; its input with a one-pixel halo is staged in
+// LDS with an 80-byte pixel stride (the 16 lanes of a 16-byte read then hit
+// 16 disjoint 4-bank groups).  Forward: D[co][pixel] = W_t[co][ci] x
+// X_t[ci][pixel] summed over the 9 taps; each lane keeps its weight fragments
+// (2 x 9 x 8 bf16) in registers for the whole block.  The input gradient is
+// the same kernel with the taps flipped and the channel roles swapped
+// (packed by k_pack), the ReLU mask of the layer input fused into its
+// epilogue.  The weight gradient is D[co][ci] = dZ[co][pixel] x A_t[pixel][ci]
+// with K running over the pixels of a tile row; dZ and three pre-shifted
+// copies of A (one per kx, so every 16-byte read is aligned) are staged
+// transposed in LDS.  Blocks loop over tiles and write per-block partial sums
+// that k_reduce adds in a fixed order: results are bitwise repeatable.
+//
+// The 1 -> 32 and 32 -> 1 layers (288 multiply-adds per pixel) run on the
+// vector ALUs.
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <utility>
+
+#include "gr_fft.hpp"
+#include "gr_internal.h"
+#include "../../include/gridrec_cnn.h"
+
+#pragma clang fp contract(off)
+
+int gr_set_error(int code, const char *msg);
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int CH = 32;
+constexpr int TH = 8, TW = 32;            // output tile (rows x columns)
+constexpr int HS = TH + 2, WS = TW + 2;   // with the halo
+constexpr int PSTR = 40;                  // LDS pixel stride in bf16 (80 bytes)
+constexpr int OFF_W1 = 0, OFF_B1 = 288, OFF_W2 = 320, OFF_B2 = OFF_W2 + 9216, OFF_W3 = OFF_B2 + 32,
+              OFF_B3 = OFF_W3 + 9216, OFF_W4 = OFF_B3 + 32, OFF_B4 = OFF_W4 + 9216, OFF_W5 = OFF_B4 + 32,
+              OFF_B5 = OFF_W5 + 288, N_PARAMS = OFF_B5 + 1;
+constexpr int PACK_LAYER = 2 * 9 * CH * CH;  // forward + flipped, bf16, per 32->32 layer
+constexpr int WG_REC = 9 * CH * CH + CH;     // weight-gradient partial record of a 32->32 layer
+constexpr int IN_REC = 9 * CH + CH;          // of the 1->32 layer
+constexpr int OUT_REC = 9 * CH + 1;          // of the 32->1 layer
+constexpr int MAX_WG_BLOCKS = 512;
+
+static_assert(N_PARAMS == 28353, "parameter layout");
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    // round to nearest, ties to even (finite inputs)
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
+__device__ __forceinline__ bool positive_bf16(uint16_t h) { return !(h & 0x8000u) && h != 0; }
+
+struct TileMap {
+    int tiles_x, tiles_y;
+    __device__ void of(int tile, int &img, int &y0, int &x0) const {
+        const int per = tiles_x * tiles_y;
+        img = tile / per;
+        const int r = tile - img * per;
+        y0 = (r / tiles_x) * TH;
+        x0 = (r % tiles_x) * TW;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// 32 -> 32 convolution on MFMA.  MODE 0: y = bf16(relu(acc + bias));
+// MODE 1 (input gradient): y = bf16(acc * (mask > 0)).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, const uint16_t *__restrict__ wp,
+                                                const float *__restrict__ bias, const uint16_t *__restrict__ mask,
+                                                uint16_t *__restrict__ y, int H, int W, TileMap tm) {
+    __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
+    int img, y0, x0;
+    tm.of(blockIdx.x, img, y0, x0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long ibase = (long)img * H * W;
+    for (int i = tid; i < HS * WS * 4; i += 256) {
+        const int pix = i >> 2, ch = i & 3;
+        const int r = pix / WS, c = pix - r * WS;
+        const int gy = y0 + r - 1, gx = x0 + c - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+            v = *reinterpret_cast<const uint4 *>(x + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+        *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
+    }
+    // weight fragments: lane holds row co = nb*16 + (lane & 15), k = ci = 8*(lane >> 4) + e
+    bf16x8 A[2][9];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
+    __syncthreads();
+
+    const int j = lane & 15, g = lane >> 4;
+#pragma unroll 1
+    for (int q = 0; q < 4; q++) {
+        const int r = wave * 2 + (q >> 1), cb = (q & 1) * 16;
+        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int ky = t / 3, kx = t - ky * 3;
+            const bf16x8 B = *reinterpret_cast<const bf16x8 *>(sx + ((r + ky) * WS + cb + j + kx) * PSTR + 8 * g);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][t], B, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][t], B, acc1, 0, 0, 0);
+        }
+        // lane holds D[co = nb*16 + 4g + e][pixel j]
+        const int gy = y0 + r, gx = x0 + cb + j;
+        if (gy < H && gx < W) {
+            const long p = ibase + (long)gy * W + gx;
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                const floatx4 a = nb ? acc1 : acc0;
+                const int co0 = nb * 16 + 4 * g;
+                float v[4] = {a[0], a[1], a[2], a[3]};
+                if (MODE == 0) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const float z = v[e] + bias[co0 + e];
+                        v[e] = z > 0.0f ? z : 0.0f;
+                    }
+                } else {
+                    const uint2 m = *reinterpret_cast<const uint2 *>(mask + p * CH + co0);
+                    const uint16_t mm[4] = {(uint16_t)(m.x & 0xffff), (uint16_t)(m.x >> 16), (uint16_t)(m.y & 0xffff),
+                                            (uint16_t)(m.y >> 16)};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) v[e] = positive_bf16(mm[e]) ? v[e] : 0.0f;
+                }
+                uint2 o;
+                o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                *reinterpret_cast<uint2 *>(y + p * CH + co0) = o;
+            }
+        }
+    }
+}
+
+// Weight gradient of a 32 -> 32 layer: partial[block] = (dW[t][co][ci], db[co])
+// over the tiles blockIdx.x, blockIdx.x + gridDim.x, ...
+constexpr int DZ_CO = TH * TW + 8;     // sdz channel stride (bf16)
+constexpr int SA_CI = HS * TW + 8;     // sa channel stride (bf16)
+constexpr size_t WGRAD_LDS = sizeof(uint16_t) * (CH * DZ_CO + 3 * CH * SA_CI) + sizeof(float) * 256 * 8;
+
+__global__ void __launch_bounds__(256) k_conv32_wgrad(const uint16_t *__restrict__ a, const uint16_t *__restrict__ dz,
+                                                      float *__restrict__ part, int H, int W, int ntiles, TileMap tm) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    uint16_t *sdz = reinterpret_cast<uint16_t *>(smem_raw);      // [co][r*TW + c]
+    uint16_t *sa = sdz + CH * DZ_CO;                             // [kx][ci][r*TW + c]
+    float *sdb = reinterpret_cast<float *>(sa + 3 * CH * SA_CI); // [256][8]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int j = lane & 15, g = lane >> 4;
+    floatx4 acc[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    float db[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int ch = tid & 3;  // the channel chunk this thread stages (256 % 4 == 0)
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();  // previous tile's reads done
+        for (int i = tid; i < TH * TW * 4; i += 256) {
+            const int pix = i >> 2;
+            const int r = pix / TW, c = pix - r * TW;
+            const int gy = y0 + r, gx = x0 + c;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gy < H && gx < W) v = *reinterpret_cast<const uint4 *>(dz + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const uint16_t h = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+                sdz[(ch * 8 + e) * DZ_CO + pix] = h;
+                db[e] += from_bf16(h);
+            }
+        }
+        for (int i = tid; i < HS * WS * 4; i += 256) {
+            const int pix = i >> 2;
+            const int r = pix / WS, cc = pix - r * WS;
+            const int gy = y0 + r - 1, gx = x0 + cc - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v = *reinterpret_cast<const uint4 *>(a + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                const int c = cc - kx;  // sa[kx] column c holds input column x0 + c + kx - 1
+                if (c < 0 || c >= TW) continue;
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    sa[(kx * CH + ch * 8 + e) * SA_CI + r * TW + c] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int r = 0; r < TH; r++) {
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const int b = wave * 9 + i;  // 36 blocks: tap t, co half, ci half
+                const int t = b >> 2, nbo = (b >> 1) & 1, nbi = b & 1;
+                const int ky = t / 3, kx = t - ky * 3;
+                const bf16x8 Av = *reinterpret_cast<const bf16x8 *>(sdz + (nbo * 16 + j) * DZ_CO + r * TW + 8 * g);
+                const bf16x8 Bv =
+                    *reinterpret_cast<const bf16x8 *>(sa + (kx * CH + nbi * 16 + j) * SA_CI + (r + ky) * TW + 8 * g);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Av, Bv, acc[i], 0, 0, 0);
+            }
+        }
+    }
+    float *out = part + (long)blockIdx.x * WG_REC;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int b = wave * 9 + i;
+        const int t = b >> 2, nbo = (b >> 1) & 1, nbi = b & 1;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int co = nbo * 16 + 4 * g + e, ci = nbi * 16 + j;
+            out[(t * CH + co) * CH + ci] = acc[i][e];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < 8; e++) sdb[tid * 8 + e] = db[e];
+    __syncthreads();
+    if (tid < CH) {  // channel tid: chunk tid/8, element tid%8, summed over the threads of that chunk in order
+        float s = 0.0f;
+        for (int k = tid >> 3; k < 256; k += 4) s += sdb[k * 8 + (tid & 7)];
+        out[9 * CH * CH + tid] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1 -> 32 layer: a1 = bf16(relu(W1 * bf16(x/scale) + b1)), one thread per pixel.
+__global__ void __launch_bounds__(256) k_conv_in(const float *__restrict__ x, float scale,
+                                                 const float *__restrict__ prm, uint16_t *__restrict__ a1, int n,
+                                                 int H, int W) {
+    __shared__ float sw[9 * CH + CH];
+    for (int i = threadIdx.x; i < 9 * CH; i += blockDim.x) sw[i] = round_bf16(prm[OFF_W1 + i]);
+    if (threadIdx.x < CH) sw[9 * CH + threadIdx.x] = prm[OFF_B1 + threadIdx.x];
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)n * H * W) return;
+    const long img = p / ((long)H * W);
+    const int rem = (int)(p - img * H * W), py = rem / W, px = rem - py * W;
+    float xv[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+        xv[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? round_bf16(x[(img * H + yy) * W + xx] / scale) : 0.0f;
+    }
+    uint32_t packed[16];
+#pragma unroll
+    for (int co = 0; co < CH; co++) {
+        float z = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) z += sw[co * 9 + t] * xv[t];
+        z += sw[9 * CH + co];
+        const uint16_t h = to_bf16(z > 0.0f ? z : 0.0f);
+        if (co & 1) packed[co >> 1] |= (uint32_t)h << 16;
+        else packed[co >> 1] = h;
+    }
+    uint4 *dst = reinterpret_cast<uint4 *>(a1 + p * CH);
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
+}
+
+// Weight gradient of the 1 -> 32 layer: partial[block] = (dW1[co][t], db1[co]).
+__global__ void __launch_bounds__(320) k_conv_in_wgrad(const float *__restrict__ x, float scale,
+                                                       const uint16_t *__restrict__ dz, float *__restrict__ part,
+                                                       int H, int W, int ntiles, TileMap tm) {
+    __shared__ float sx[HS * WS];
+    __shared__ float sdz[TH * TW * (CH + 1)];
+    const int tid = threadIdx.x;
+    float acc = 0.0f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();
+        for (int i = tid; i < HS * WS; i += 320) {
+            const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
+            sx[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
+        }
+        for (int i = tid; i < TH * TW * CH; i += 320) {
+            const int pix = i / CH, co = i - pix * CH, r = pix / TW, c = pix - r * TW;
+            const int gy = y0 + r, gx = x0 + c;
+            sdz[pix * (CH + 1) + co] = (gy < H && gx < W) ? from_bf16(dz[(ibase + (long)gy * W + gx) * CH + co]) : 0.0f;
+        }
+        __syncthreads();
+        if (tid < 9 * CH) {
+            const int co = tid / 9, t = tid - co * 9, ky = t / 3, kx = t - ky * 3;
+            for (int r = 0; r < TH; r++)
+                for (int c = 0; c < TW; c++) acc += sdz[(r * TW + c) * (CH + 1) + co] * sx[(r + ky) * WS + c + kx];
+        } else if (tid < IN_REC) {
+            const int co = tid - 9 * CH;
+            for (int pix = 0; pix < TH * TW; pix++) acc += sdz[pix * (CH + 1) + co];
+        }
+    }
+    if (tid < IN_REC) part[(long)blockIdx.x * IN_REC + tid] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// 32 -> 1 layer with the residual: r = W5 * a4 + b5, yn = x/scale + r,
+// y = yn * scale; per-tile sums of r^2 and (x/scale)^2 for the fallback test.
+__global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a4, const float *__restrict__ x,
+                                                  float scale, const float *__restrict__ prm, float *__restrict__ y,
+                                                  float *__restrict__ yn, float *__restrict__ stats, int H, int W,
+                                                  TileMap tm) {
+    __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
+    __shared__ float sw[9 * CH + 1];
+    __shared__ float sred[2][256];
+    int img, y0, x0;
+    tm.of(blockIdx.x, img, y0, x0);
+    const int tid = threadIdx.x;
+    const long ibase = (long)img * H * W;
+    for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W5 + i]);
+    if (tid == 0) sw[9 * CH] = prm[OFF_B5];
+    for (int i = tid; i < HS * WS * 4; i += 256) {
+        const int pix = i >> 2, ch = i & 3;
+        const int r = pix / WS, c = pix - r * WS;
+        const int gy = y0 + r - 1, gx = x0 + c - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
+            v = *reinterpret_cast<const uint4 *>(a4 + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+        *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
+    }
+    __syncthreads();
+    const int r = tid / TW, c = tid - r * TW;
+    const int gy = y0 + r, gx = x0 + c;
+    float rr = 0.0f, xx = 0.0f;
+    if (gy < H && gx < W) {
+        float s = 0.0f;
+        for (int t = 0; t < 9; t++) {
+            const uint16_t *px = sx + ((r + t / 3) * WS + c + t % 3) * PSTR;
+            for (int ci = 0; ci < CH; ci++) s += sw[ci * 9 + t] * from_bf16(px[ci]);
+        }
+        s += sw[9 * CH];
+        const long p = ibase + (long)gy * W + gx;
+        const float xn = x[p] / scale;
+        const float v = xn + s;
+        if (yn) yn[p] = v;
+        if (y) y[p] = v * scale;
+        rr = s * s;
+        xx = xn * xn;
+    }
+    sred[0][tid] = rr;
+    sred[1][tid] = xx;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            sred[0][tid] += sred[0][tid + h];
+            sred[1][tid] += sred[1][tid + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && stats) {
+        stats[2 * blockIdx.x] = sred[0][0];
+        stats[2 * blockIdx.x + 1] = sred[1][0];
+    }
+}
+
+// Loss: dy = 2 (yn - t/scale) / N; per-block partial sums of (yn - t/scale)^2.
+__global__ void __launch_bounds__(256) k_loss(const float *__restrict__ yn, const float *__restrict__ target,
+                                              float scale, float *__restrict__ dy, float *__restrict__ part, long N) {
+    __shared__ float sred[256];
+    float s = 0.0f;
+    for (long p = (long)blockIdx.x * 256 + threadIdx.x; p < N; p += (long)gridDim.x * 256) {
+        const float e = yn[p] - target[p] / scale;
+        dy[p] = 2.0f * e / (float)N;
+        s += e * e;
+    }
+    sred[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) sred[threadIdx.x] += sred[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sred[0];
+}
+
+// Input gradient of the 32 -> 1 layer, masked: dz4 = bf16(sum_t dy[p - off_t] W5[ci][t] * (a4 > 0)).
+__global__ void __launch_bounds__(256) k_conv_out_dgrad(const float *__restrict__ dy, const uint16_t *__restrict__ a4,
+                                                        const float *__restrict__ prm, uint16_t *__restrict__ dz4,
+                                                        int n, int H, int W) {
+    __shared__ float sw[9 * CH];
+    for (int i = threadIdx.x; i < 9 * CH; i += blockDim.x) sw[i] = round_bf16(prm[OFF_W5 + i]);
+    __syncthreads();
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)n * H * W) return;
+    const long img = p / ((long)H * W);
+    const int rem = (int)(p - img * H * W), py = rem / W, px = rem - py * W;
+    float d[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const int yy = py - (t / 3 - 1), xx = px - (t % 3 - 1);
+        d[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? dy[(img * H + yy) * W + xx] : 0.0f;
+    }
+    const uint4 *m4 = reinterpret_cast<const uint4 *>(a4 + p * CH);
+    uint4 *o4 = reinterpret_cast<uint4 *>(dz4 + p * CH);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint4 m = m4[k];
+        const uint32_t mw[4] = {m.x, m.y, m.z, m.w};
+        uint32_t ow[4];
+#pragma unroll
+        for (int e2 = 0; e2 < 4; e2++) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int ci = k * 8 + e2 * 2 + h;
+                float s = 0.0f;
+#pragma unroll
+                for (int t = 0; t < 9; t++) s += d[t] * sw[ci * 9 + t];
+                const uint16_t mk = (uint16_t)(mw[e2] >> (16 * h));
+                word |= (uint32_t)to_bf16(positive_bf16(mk) ? s : 0.0f) << (16 * h);
+            }
+            ow[e2] = word;
+        }
+        o4[k] = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+    }
+}
+
+// Weight gradient of the 32 -> 1 layer: partial[block] = (dW5[ci][t], db5).
+__global__ void __launch_bounds__(320) k_conv_out_wgrad(const float *__restrict__ dy, const uint16_t *__restrict__ a4,
+                                                        float *__restrict__ part, int H, int W, int ntiles,
+                                                        TileMap tm) {
+    __shared__ float sdy[TH * TW];
+    __shared__ float sa[HS * WS * (CH + 1)];
+    const int tid = threadIdx.x;
+    float acc = 0.0f;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();
+        for (int i = tid; i < TH * TW; i += 320) {
+            const int r = i / TW, c = i - r * TW, gy = y0 + r, gx = x0 + c;
+            sdy[i] = (gy < H && gx < W) ? dy[ibase + (long)gy * W + gx] : 0.0f;
+        }
+        for (int i = tid; i < HS * WS * CH; i += 320) {
+            const int pix = i / CH, ci = i - pix * CH, r = pix / WS, c = pix - r * WS;
+            const int gy = y0 + r - 1, gx = x0 + c - 1;
+            sa[pix * (CH + 1) + ci] =
+                (gy >= 0 && gy < H && gx >= 0 && gx < W) ? from_bf16(a4[(ibase + (long)gy * W + gx) * CH + ci]) : 0.0f;
+        }
+        __syncthreads();
+        if (tid < 9 * CH) {
+            const int ci = tid / 9, t = tid - ci * 9, ky = t / 3, kx = t - ky * 3;
+            for (int r = 0; r < TH; r++)
+                for (int c = 0; c < TW; c++) acc += sdy[r * TW + c] * sa[((r + ky) * WS + c + kx) * (CH + 1) + ci];
+        } else if (tid < OUT_REC) {
+            for (int i = 0; i < TH * TW; i++) acc += sdy[i];
+        }
+    }
+    if (tid < OUT_REC) part[(long)blockIdx.x * OUT_REC + tid] = acc;
+}
+
+// Sum of nparts partial records in block order (double) into the gradient
+// vector.  kind 0: 32->32 record (t,co,ci)+db -> W[co][ci][t], b; kind 1 and 2
+// records are already in parameter order.
+__global__ void k_reduce(const float *__restrict__ part, int nparts, int rec, int kind, float *__restrict__ gw,
+                         float *__restrict__ gb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rec) return;
+    double s = 0.0;
+    for (int k = 0; k < nparts; k++) s += (double)part[(long)k * rec + i];
+    const float v = (float)s;
+    if (kind == 0) {
+        if (i < 9 * CH * CH) {
+            const int t = i / (CH * CH), co = (i / CH) % CH, ci = i % CH;
+            gw[(co * CH + ci) * 9 + t] = v;
+        } else {
+            gb[i - 9 * CH * CH] = v;
+        }
+    } else {
+        const int nw = 9 * CH;
+        if (i < nw) gw[i] = v;
+        else gb[i - nw] = v;
+    }
+}
+
+// Sum of per-block partials: out[k] = sum of part[k*per + i], i < per (in order).
+__global__ void k_sum_groups(const float *__restrict__ part, int groups, int per, int stride, float *__restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= groups * stride) return;
+    const int grp = k / stride, comp = k - grp * stride;
+    double s = 0.0;
+    for (int i = 0; i < per; i++) s += (double)part[((long)grp * per + i) * stride + comp];
+    out[k] = (float)s;
+}
+
+// Packed bf16 weights of the three 32->32 layers: forward [t][co][ci] and
+// input-gradient [t][ci][co] = W[co][ci][8 - t].
+__global__ void k_pack(const float *__restrict__ prm, uint16_t *__restrict__ packed) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * 9 * CH * CH) return;
+    const int l = i / (9 * CH * CH), k = i - l * 9 * CH * CH;
+    const int t = k / (CH * CH), a = (k / CH) % CH, b = k % CH;
+    const float *w = prm + OFF_W2 + l * (9 * CH * CH + CH);
+    uint16_t *fw = packed + (long)l * PACK_LAYER, *bw = fw + 9 * CH * CH;
+    fw[k] = to_bf16(w[(a * CH + b) * 9 + t]);      // a = co, b = ci
+    bw[k] = to_bf16(w[(b * CH + a) * 9 + 8 - t]);  // a = ci (output), b = co (input)
+}
+
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+int cnn_fail(int code, const char *msg) { return gr_set_error(code, msg); }
+#define CNN_HIP(call)                                                       \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) return cnn_fail(GR_EHIP, hipGetErrorString(e_)); \
+    } while (0)
+
+TileMap tiles_of(int H, int W) { return TileMap{(W + TW - 1) / TW, (H + TH - 1) / TH}; }
+
+int check_dims(int n, int H, int W) {
+    if (n < 1 || H < 1 || W < 1 || (long)n * H * W > (1L << 31) - 1) return cnn_fail(GR_EINVAL, "invalid image batch");
+    const TileMap tm = tiles_of(H, W);
+    if ((long)tm.tiles_x * tm.tiles_y * n > (1L << 31) - 1) return cnn_fail(GR_EINVAL, "too many tiles");
+    return GR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+long gr_cnn_param_count(void) { return N_PARAMS; }
+long gr_cnn_packed_count(void) { return 3L * PACK_LAYER; }
+
+size_t gr_cnn_work_bytes(int n, int H, int W) {
+    const TileMap tm = tiles_of(H, W);
+    const long ntiles = (long)tm.tiles_x * tm.tiles_y * n;
+    const long npx = (long)n * H * W;
+    const size_t up = 256;
+    auto al = [&](size_t b) { return (b + up - 1) / up * up; };
+    return al(sizeof(float) * npx)                         // yn
+           + al(sizeof(float) * npx)                       // dy
+           + 2 * al(sizeof(uint16_t) * npx * CH)           // dz ping-pong
+           + al(sizeof(float) * (size_t)MAX_WG_BLOCKS * WG_REC)  // weight-gradient partials
+           + al(sizeof(float) * 2 * (size_t)ntiles)        // forward statistics partials
+           + al(sizeof(float) * 1024);                     // loss partials
+}
+
+int gr_cnn_pack(const float *params, unsigned short *packed, void *stream) {
+    if (!params || !packed) return cnn_fail(GR_EINVAL, "null argument");
+    const int n = 3 * 9 * CH * CH;
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, params, packed);
+    CNN_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+int gr_cnn_forward(const float *params, const unsigned short *packed, const float *x, int n, int H, int W, float scale,
+                   unsigned short *a1, unsigned short *a2, unsigned short *a3, unsigned short *a4, float *y,
+                   float *yn, float *slice_stats, void *work, void *stream) {
+    if (!params || !packed || !x || !a1 || !a2 || !a3 || !a4 || !(scale > 0.0f)) return cnn_fail(GR_EINVAL, "null argument or scale <= 0");
+    if (a1 == a2 || a2 == a3 || a3 == a4) return cnn_fail(GR_EINVAL, "consecutive activations must not alias");
+    int rc = check_dims(n, H, W);
+    if (rc) return rc;
+    if (slice_stats && !work) return cnn_fail(GR_EINVAL, "slice statistics need the work buffer");
+    hipStream_t s = (hipStream_t)stream;
+    const TileMap tm = tiles_of(H, W);
+    const int ntiles = tm.tiles_x * tm.tiles_y * n;
+    const long npx = (long)n * H * W;
+    const uint16_t *pk = reinterpret_cast<const uint16_t *>(packed);
+    hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, x, scale, params, a1, n, H, W);
+    uint16_t *acts[4] = {a1, a2, a3, a4};
+    const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
+    for (int l = 0; l < 3; l++)
+        hipLaunchKernelGGL(k_conv32<0>, dim3(ntiles), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
+                           params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, tm);
+    float *stats = nullptr;
+    if (slice_stats) {
+        char *wb = (char *)work;
+        auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+        wb += 2 * al(sizeof(float) * npx) + 2 * al(sizeof(uint16_t) * npx * CH) +
+              al(sizeof(float) * (size_t)MAX_WG_BLOCKS * WG_REC);
+        stats = (float *)wb;
+    }
+    hipLaunchKernelGGL(k_conv_out, dim3(ntiles), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W, tm);
+    if (slice_stats) {
+        const int per = tm.tiles_x * tm.tiles_y;
+        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, per, 2, slice_stats);
+    }
+    CNN_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+int gr_cnn_backward(const float *params, const unsigned short *packed, const float *x, const float *target, int n,
+                    int H, int W, float scale, const unsigned short *a1, const unsigned short *a2,
+                    const unsigned short *a3, const unsigned short *a4, const float *yn, float *grads, float *loss,
+                    void *work, void *stream) {
+    if (!params || !packed || !x || !target || !a1 || !a2 || !a3 || !a4 || !yn || !grads || !work || !(scale > 0.0f))
+        return cnn_fail(GR_EINVAL, "null argument or scale <= 0");
+    int rc = check_dims(n, H, W);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const TileMap tm = tiles_of(H, W);
+    const int ntiles = tm.tiles_x * tm.tiles_y * n;
+    const long npx = (long)n * H * W;
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    char *wb = (char *)work;
+    wb += al(sizeof(float) * npx);  // yn slot (the caller's yn may live there)
+    float *dy = (float *)wb;
+    wb += al(sizeof(float) * npx);
+    uint16_t *dzA = (uint16_t *)wb;
+    wb += al(sizeof(uint16_t) * npx * CH);
+    uint16_t *dzB = (uint16_t *)wb;
+    wb += al(sizeof(uint16_t) * npx * CH);
+    float *part = (float *)wb;
+    wb += al(sizeof(float) * (size_t)MAX_WG_BLOCKS * WG_REC);
+    wb += al(sizeof(float) * 2 * (size_t)ntiles);
+    float *lpart = (float *)wb;
+    const uint16_t *pk = reinterpret_cast<const uint16_t *>(packed);
+    const int nwg = ntiles < MAX_WG_BLOCKS ? ntiles : MAX_WG_BLOCKS;
+    const int nloss = (int)std::min<long>(1024, (npx + 255) / 256);
+
+    hipLaunchKernelGGL(k_loss, dim3(nloss), dim3(256), 0, s, yn, target, scale, dy, lpart, npx);
+    if (loss) hipLaunchKernelGGL(k_sum_groups, dim3(1), dim3(64), 0, s, lpart, 1, nloss, 1, loss);
+    // layer 5
+    hipLaunchKernelGGL(k_conv_out_wgrad, dim3(nwg), dim3(320), 0, s, dy, a4, part, H, W, ntiles, tm);
+    hipLaunchKernelGGL(k_reduce, dim3((OUT_REC + 255) / 256), dim3(256), 0, s, part, nwg, OUT_REC, 1,
+                       grads + OFF_W5, grads + OFF_B5);
+    hipLaunchKernelGGL(k_conv_out_dgrad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, dy, a4, params, dzA,
+                       n, H, W);
+    // layers 4, 3, 2
+    const unsigned short *acts[4] = {a1, a2, a3, a4};
+    const int woff[3] = {OFF_W2, OFF_W3, OFF_W4}, boff[3] = {OFF_B2, OFF_B3, OFF_B4};
+    CNN_HIP(gr_allow_lds(k_conv32_wgrad, WGRAD_LDS));
+    uint16_t *dz = dzA, *dzn = dzB;
+    for (int l = 2; l >= 0; l--) {  // layer l+2: input acts[l], output gradient dz
+        const uint16_t *ain = reinterpret_cast<const uint16_t *>(acts[l]);
+        hipLaunchKernelGGL(k_conv32_wgrad, dim3(nwg), dim3(256), WGRAD_LDS, s, ain, dz, part, H, W, ntiles, tm);
+        hipLaunchKernelGGL(k_reduce, dim3((WG_REC + 255) / 256), dim3(256), 0, s, part, nwg, WG_REC, 0,
+                           grads + woff[l], grads + boff[l]);
+        hipLaunchKernelGGL(k_conv32<1>, dim3(ntiles), dim3(256), 0, s, dz, pk + (long)l * PACK_LAYER + 9 * CH * CH,
+                           (const float *)nullptr, ain, dzn, H, W, tm);
+        std::swap(dz, dzn);
+    }
+    // layer 1
+    hipLaunchKernelGGL(k_conv_in_wgrad, dim3(nwg), dim3(320), 0, s, x, scale, dz, part, H, W, ntiles, tm);
+    hipLaunchKernelGGL(k_reduce, dim3((IN_REC + 255) / 256), dim3(256), 0, s, part, nwg, IN_REC, 1,
+                       grads + OFF_W1, grads + OFF_B1);
+    CNN_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+int gr_cnn_adam(float *params, const float *grads, float *m, float *v, long count, int step, float lr, float beta1,
+                float beta2, float eps, void *stream) {
+    if (!params || !grads || !m || !v || count < 1 || count > (1L << 30) || step < 1)
+        return cnn_fail(GR_EINVAL, "invalid Adam arguments");
+    const float c1 = (float)(1.0 / (1.0 - pow((double)beta1, step)));
+    const float c2 = (float)(1.0 / (1.0 - pow((double)beta2, step)));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params,
+                       grads, m, v, (int)count, lr, beta1, beta2, eps, c1, c2);
+    CNN_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+}  // extern "C"

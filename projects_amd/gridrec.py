@@ -70,7 +70,7 @@ class GrQueueInfo(C.Structure):
     ]
 
 
-# every function of include/gridrec_hip.h with its ctypes signature
+# every function of include/gridrec_hip.h and include/gridrec_cnn.h with its ctypes signature
 _SIGS = {
     "gr_get_pswf": (C.c_int, [C.c_float, C.POINTER(GrPswf)]),
     "gr_filter_lookup": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_int)]),
@@ -93,6 +93,17 @@ _SIGS = {
     "gr_queue_dataset_settings": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int)]),
     "gr_queue_destroy": (C.c_int, [C.c_void_p]),
+    # include/gridrec_cnn.h
+    "gr_cnn_param_count": (C.c_long, []),
+    "gr_cnn_packed_count": (C.c_long, []),
+    "gr_cnn_work_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "gr_cnn_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "gr_cnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float]
+                       + [C.c_void_p] * 9),
+    "gr_cnn_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                  C.c_float] + [C.c_void_p] * 9),
+    "gr_cnn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_int, C.c_float,
+                              C.c_float, C.c_float, C.c_float, C.c_void_p]),
     "gr_last_error": (C.c_char_p, []),
     "gr_version": (C.c_char_p, []),
 }

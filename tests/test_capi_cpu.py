@@ -1,7 +1,7 @@
 """CPU checks of the C-ABI library (no GPU calls that compute).
 
 * the library loads and exports every function declared in include/*.h
-  (C names from gridrec_hip.h, C++ names of the drop-in grid.h);
+  (C names from gridrec_hip.h and gridrec_cnn.h, C++ names of the drop-in grid.h);
 * argument validation returns the documented status codes before any device
   work: unknown PSWF C (pswf.cpp:39-44), geom outside {0,1,2} (grid.cpp:557-560),
   non-positive sizes, a crop window outside the image;
@@ -21,8 +21,9 @@ ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
 
 
 def declared_c_functions():
-    text = open(os.path.join(ROOT, "include", "gridrec_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|char|void)\s*\*?\s*(gr_\w+)\s*\(", text, re.M)))
+    text = "".join(open(os.path.join(ROOT, "include", h)).read() for h in ("gridrec_hip.h", "gridrec_cnn.h"))
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?(?:int|char|void|long|size_t)\s*\*?\s*(gr_\w+)\s*\(", text,
+                                 re.M)))
 
 
 def exported():

@@ -1,0 +1,152 @@
+"""GPU checks of the CNN kernels (k_cnn.hip) against the CPU oracle.
+
+Bars:
+* each layer, fed the GPU's own BF16 input, reproduces the oracle's BF16
+  output to within one BF16 unit in the last place (FP32 accumulation order
+  may move a value across a rounding boundary) on at most 1% of the values;
+* the FP32 output y and every parameter gradient agree with the oracle
+  end to end (the BF16 activation chain of each side) to rel-L2 1e-2, the loss
+  to 1e-5;
+* Adam agrees with the oracle's update to FP32 rounding;
+* a few hundred Adam steps on a fixed task lower the loss.
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as K
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from projects_amd import learned as Lrn
+
+
+def bf16_bits_to_f32(t):
+    u = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return u.view(np.float32)
+
+
+def ulp_diff(a, b):
+    """Distance in BF16 units between two arrays of BF16 values (as float32)."""
+    ia = (a.view(np.uint32) >> 16).astype(np.int64)
+    ib = (b.view(np.uint32) >> 16).astype(np.int64)
+    ia = np.where(ia & 0x8000, -(ia & 0x7FFF), ia)
+    ib = np.where(ib & 0x8000, -(ib & 0x7FFF), ib)
+    return np.abs(ia - ib)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def make(seed, n=2, H=19, W=45, scale=1.7):
+    rng = np.random.default_rng(seed)
+    p = Lrn.init_params(seed)
+    pd = Lrn.unflatten(p)
+    pd["W5"] = pd["W5"] * 5.0
+    for k in ("b1", "b2", "b3", "b4", "b5"):
+        pd[k][...] = rng.standard_normal(pd[k].shape) * 0.05
+    flat = np.concatenate([pd[k].ravel() for k, _ in Lrn.PARAM_SHAPES]).astype(np.float32)
+    x = rng.standard_normal((n, H, W)).astype(np.float32) * 2.0
+    t = (x + rng.standard_normal((n, H, W)).astype(np.float32) * 0.3).astype(np.float32)
+    net = Lrn.Corrector(params=flat, scale=scale)
+    return net, {k: v.astype(np.float32) for k, v in pd.items()}, x, t
+
+
+def test_forward_layers_match_oracle():
+    net, pd, x, _ = make(1)
+    xg = torch.from_numpy(x).cuda()
+    y, stats = net.forward(xg, train=True)
+    torch.cuda.synchronize()
+    acts = net._buffers(*x.shape, True)[0]
+    a = [bf16_bits_to_f32(t) for t in acts]
+    # layer 1 from x; layers 2-4 from the GPU's previous activation
+    c = K.forward(pd, x, net.scale)
+    wq = c["wq"]
+    expect = [c["a1"]]
+    for l in (2, 3, 4):
+        z = K.conv3x3(a[l - 2], wq[f"W{l}"]) + wq[f"b{l}"]
+        expect.append(K.bf16(np.maximum(z, 0.0)))
+    for l in range(4):
+        e = expect[l].astype(np.float32)
+        d = ulp_diff(a[l], e)
+        print(f"a{l + 1}: max ulp {d.max()}, off {np.mean(d > 0):.2e}", flush=True)
+        assert d.max() <= 1 and np.mean(d > 0) < 0.01, l
+    r = K.conv3x3(a[3], wq["W5"][None])[..., 0] + wq["b5"][0]
+    yn = x / np.float32(net.scale) + r
+    assert rel(y.cpu().numpy(), yn * net.scale) < 1e-5
+    s = stats.cpu().numpy()
+    assert np.allclose(s[:, 0], (r ** 2).reshape(2, -1).sum(1), rtol=1e-4)
+    assert np.allclose(s[:, 1], ((x / np.float32(net.scale)).astype(np.float64) ** 2).reshape(2, -1).sum(1),
+                       rtol=1e-4)
+    # end to end against the oracle's own BF16 chain
+    assert rel(y.cpu().numpy(), c["y"] * net.scale) < 1e-2
+
+
+def test_backward_matches_oracle():
+    net, pd, x, t = make(2)
+    xg, tg = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
+    net.forward(xg, train=True)
+    net.backward(xg, tg)
+    torch.cuda.synchronize()
+    acts = [bf16_bits_to_f32(a) for a in net._buffers(*x.shape, True)[0]]
+    # oracle backward on the GPU's activations: differences are accumulation order
+    c = K.forward(pd, x, net.scale)
+    for l in range(4):
+        c[f"a{l + 1}"] = acts[l]
+    yn = net._buffers(*x.shape, True)[2].cpu().numpy()
+    c["y"] = yn
+    loss, g = K.loss_and_backward(pd, c, t, net.scale)
+    assert abs(float(net.loss.cpu()) - loss) <= 1e-5 * loss
+    gg = Lrn.unflatten(net.grads.cpu().numpy())
+    for k in g:
+        e = rel(gg[k], g[k])
+        print(f"grad {k}: rel-L2 {e:.2e}", flush=True)
+        assert e < 1e-2, k
+
+
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(3)
+    net = Lrn.Corrector(seed=3)
+    p0 = net.params.cpu().numpy().astype(np.float64)
+    pd = {"p": p0.copy()}
+    m, v = {"p": np.zeros_like(p0)}, {"p": np.zeros_like(p0)}
+    for step in range(1, 4):
+        g = rng.standard_normal(p0.shape).astype(np.float32)
+        net.grads.copy_(torch.from_numpy(g))
+        net.adam(lr=1e-3)
+        pd, m, v = K.adam(pd, {"p": g.astype(np.float64)}, m, v, step)
+    torch.cuda.synchronize()
+    got = net.params.cpu().numpy()
+    assert np.max(np.abs(got - pd["p"])) < 1e-6
+
+
+def test_training_lowers_loss():
+    """Learn to remove a fixed 3x3 blur residual: the loss falls by 4x."""
+    rng = np.random.default_rng(4)
+    net = Lrn.Corrector(seed=4, scale=1.0)
+    n, H, W = 8, 32, 64
+    tgt = rng.standard_normal((n, H, W)).astype(np.float32)
+    k = np.array([[1, 2, 1], [2, 4, 2], [1, 2, 1]], np.float32) / 16.0
+    pad = np.pad(tgt, ((0, 0), (1, 1), (1, 1)))
+    x = sum(k[i, j] * pad[:, i:i + H, j:j + W] for i in range(3) for j in range(3)).astype(np.float32)
+    xg, tg = torch.from_numpy(x).cuda(), torch.from_numpy(tgt).cuda()
+    first = float(net.train_step(xg, tg, lr=2e-3).cpu())
+    for _ in range(300):
+        net.train_step(xg, tg, lr=2e-3)
+    last = float(net.train_step(xg, tg, lr=2e-3).cpu())
+    print(f"loss {first:.4f} -> {last:.4f}", flush=True)
+    assert last < first / 4
+
+
+def test_fallback_keeps_plain_image():
+    net, _, x, _ = make(5, n=3, H=16, W=32)
+    xg = torch.from_numpy(x).cuda()
+    net.threshold = 0.0  # every correction is "too large"
+    y, used = net.correct(xg)
+    torch.cuda.synchronize()
+    assert not bool(used.any()) and torch.equal(y, xg)
+    net.threshold = 1e9
+    y, used = net.correct(xg)
+    assert bool(used.all()) and not torch.equal(y, xg)
