@@ -32,6 +32,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "slices/s @2048² (1500 ang) plain vs batched vs learned; rel-L2 vs FP64, PSNR"
+
+
+def progress(msg: str):
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 HBM_PEAK_GBS = 8000.0
 
 
@@ -48,8 +53,15 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--workload", default="all", choices=["all", "plain", "batched", "precision"],
-                    help="all: the plain headline line with the batched queue and the precision study after it")
+    ap.add_argument("--workload", default="all", choices=["all", "plain", "batched", "precision", "learned"],
+                    help="all: the plain headline line with the batched queue, the precision study and the "
+                         "learned sparse-angle path after it")
+    ap.add_argument("--learned-pairs", type=int, default=20000, help="training pairs (512^2, 90 / 900 angles)")
+    ap.add_argument("--learned-seconds", type=float, default=90.0, help="training time budget")
+    ap.add_argument("--learned-steps", type=int, default=0, help="training step cap (0 = time budget only)")
+    ap.add_argument("--learned-slices", type=int, default=256, help="inference slices (2048^2, 150 angles)")
+    ap.add_argument("--learned-threshold", type=float, default=0.5,
+                    help="fallback when ||residual|| / ||image|| exceeds this")
     ap.add_argument("--precision-slices", type=int, default=256)
     ap.add_argument("--precision-fp64-pairs", type=int, default=0,
                     help="pairs checked against the FP64 oracle (0 = all)")
@@ -126,6 +138,7 @@ def bench_batched(a, gr, O, torch):
     """The batched uneven queue against per-dataset plain plans on the same
     datasets: execution time of both (HIP events), queue build time, bitwise
     equality of the images (checksums), one pair against the oracle."""
+    progress("batched: generating the uneven datasets")
     t0 = time.time()
     sets = queue_datasets(gr, torch, a.seed + 1, a.queue_slices)
     gen_s = time.time() - t0
@@ -149,6 +162,7 @@ def bench_batched(a, gr, O, torch):
         plain_ms += e0.elapsed_time(e1)
         p.close()
     ref_sums = bit_checksums(torch, sets)
+    progress(f"batched: per-dataset plain plans done ({plain_ms:.0f} ms of kernels)")
     for _, _, o in sets:
         o.zero_()
     torch.cuda.synchronize()
@@ -167,6 +181,7 @@ def bench_batched(a, gr, O, torch):
     e1.record(stream)
     e1.synchronize()
     queue_ms = e0.elapsed_time(e1) / steps
+    progress(f"batched: queue built in {build_s:.1f} s, {queue_ms:.0f} ms per batch")
     same = bit_checksums(torch, sets) == ref_sums
 
     # one pair of a small dataset against the oracle restatement and FP64
@@ -232,6 +247,7 @@ def bench_precision(a, gr, O, torch):
     torch.cuda.synchronize()
     stream = torch.cuda.Stream()
     variants = [("compensated", "reference"), ("plain", "reference"), ("compensated", "fp64"), ("plain", "fp64")]
+    progress("precision: GPU variants")
     outs, rates = {}, {}
     for acc, tab in variants:
         p = gr.Plan(n_ang=nang, n_det=ndet, center=center, accum=acc, tables=tab, pad="half", crop=crop)
@@ -273,6 +289,7 @@ def bench_precision(a, gr, O, torch):
         for k0 in range(0, n_pairs, threads):  # FP64 oracle on the host cores, a wave of pairs at a time
             ks = list(range(k0, min(n_pairs, k0 + threads)))
             ds = list(ex.map(lambda k: O.recon_f64(g, O.pad_half(s_host[2 * k]), O.pad_half(s_host[2 * k + 1])), ks))
+            progress(f"precision: FP64 oracle pairs {ks[0]}-{ks[-1]} of {n_pairs}")
             per_pair += [errors(k, d) for k, d in zip(ks, ds)]
     oracle_s = time.perf_counter() - t0
     summary = {}
@@ -311,6 +328,99 @@ def bench_precision(a, gr, O, torch):
     }
 
 
+def bench_learned(a, gr, torch):
+    """Learned sparse-angle path.  Training: 512^2 seeded phantoms (20 ellipses
+    + band-limited texture, noise-free), gridrec from 90 angles as input and
+    from 900 angles as target, Adam (lr 1e-3, batch 32) for a time budget on
+    the GPU.  Inference: 2048^2 slices from 150-angle sinograms, gridrec + the
+    network (with the fallback), judged by PSNR / rel-L2 against 1500-angle
+    plain gridrec of the same phantoms; both paths timed with HIP events."""
+    from projects_amd import learned as Lrn
+
+    res = {}
+    progress("learned: generating training pairs")
+    t0 = time.perf_counter()
+    inp, tgt = Lrn.make_pairs(a.learned_pairs, 512, 90, 900, seed=a.seed + 5)
+    progress(f"learned: {a.learned_pairs} pairs in {time.perf_counter() - t0:.1f} s; training")
+    res["train_pairs"] = a.learned_pairs
+    res["pair_gen_s"] = round(time.perf_counter() - t0, 2)
+    scale = float(tgt[:256].std())
+    net = Lrn.Corrector(seed=a.seed, scale=scale, threshold=a.learned_threshold)
+    hist, steps, train_s = Lrn.train(net, inp, tgt, steps=a.learned_steps, seconds=a.learned_seconds, batch=32,
+                                     lr=1e-3, progress=progress)
+    progress(f"learned: {steps} steps in {train_s:.1f} s; inference")
+    res.update(train_steps=steps, train_s=round(train_s, 1), epochs=round(steps * 32 / a.learned_pairs, 2),
+               train_ms_per_step=round(train_s / steps * 1e3, 2),
+               loss_first=float(f"{hist[0][1]:.4e}"), loss_last=float(f"{hist[-1][1]:.4e}"),
+               loss_curve=[(k, float(f"{v:.4e}")) for k, v in hist[:: max(1, len(hist) // 12)]])
+    del inp, tgt
+    torch.cuda.empty_cache()
+    # held-out pairs at the training size
+    vi, vt = Lrn.make_pairs(64, 512, 90, 900, seed=a.seed + 5, slice0=10_000_000)
+    vy, _ = net.correct(vi)
+    torch.cuda.synchronize()
+    res["heldout_512"] = {"psnr_plain90_vs_900": round(np.mean([Lrn.psnr(vi[i], vt[i]) for i in range(64)]), 2),
+                          "psnr_learned90_vs_900": round(np.mean([Lrn.psnr(vy[i], vt[i]) for i in range(64)]), 2)}
+    del vi, vt, vy
+
+    # inference at 2048^2: 150 angles (+ network) against 1500-angle plain gridrec
+    ns, n = a.learned_slices, 2048
+    stream = torch.cuda.Stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    outs = {}
+    for na in (150, 1500):
+        sino = torch.empty((ns, na, n), dtype=torch.float32, device="cuda")
+        for s0 in range(0, ns, 64):
+            k = min(64, ns - s0)
+            gr.phantom_sinograms(sino[s0:s0 + k], a.seed + 6, na, n, [n / 2.0] * k, 20, slice0=s0, n_tex=64,
+                                 tex_amp=0.15, tex_fmax=24.0)
+        plan = gr.Plan(n_ang=na, n_det=n, center=n / 2.0, pad="half", crop=(n // 2, n))
+        out = torch.empty((ns, n, n), dtype=torch.float32, device="cuda")
+        y = torch.empty_like(out) if na == 150 else None
+        for rep in range(2):  # warm-up, then timed
+            e0.record(stream)
+            plan.recon(sino, out, stream=stream)
+            e1.record(stream)
+            e1.synchronize()
+            t_recon = e0.elapsed_time(e1)
+            t_cnn = 0.0
+            if na == 150:
+                with torch.cuda.stream(stream):
+                    e0.record(stream)
+                    for s0 in range(0, ns, 16):  # 16 slices of 2048^2 per network launch
+                        net.correct(out[s0:s0 + 16], out=y[s0:s0 + 16], stream=stream)
+                    e1.record(stream)
+                e1.synchronize()
+                t_cnn = e0.elapsed_time(e1)
+        outs[na] = (out, y, t_recon, t_cnn)
+        plan.close()
+        del sino
+    plain, _, t1500, _ = outs[1500]
+    sparse, corr, t150, tcnn = outs[150]
+    _, used = net.correct(sparse[0:16].contiguous())
+    ps_sparse = [Lrn.psnr(sparse[i], plain[i]) for i in range(ns)]
+    ps_learned = [Lrn.psnr(corr[i], plain[i]) for i in range(ns)]
+    rl = lambda u, v: float(torch.linalg.norm((u - v).double()) / torch.linalg.norm(v.double()))
+    cnn_flops = 2.0 * (9 * 32 + 3 * 9 * 32 * 32 + 9 * 32) * n * n * ns
+    res["inference_2048"] = {
+        "slices": ns,
+        "plain1500_slices_s": round(ns / (t1500 * 1e-3), 1),
+        "learned150_slices_s": round(ns / ((t150 + tcnn) * 1e-3), 1),
+        "speedup_vs_plain1500": round(t1500 / (t150 + tcnn), 3),
+        "gridrec150_ms": round(t150, 2), "cnn_ms": round(tcnn, 2),
+        "cnn_TFLOPs": round(cnn_flops / (tcnn * 1e-3) / 1e12, 1),
+        "psnr_plain150_vs_plain1500": round(float(np.mean(ps_sparse)), 2),
+        "psnr_learned150_vs_plain1500": round(float(np.mean(ps_learned)), 2),
+        "rel_l2_plain150_vs_plain1500": float(f"{np.mean([rl(sparse[i], plain[i]) for i in range(ns)]):.4e}"),
+        "rel_l2_learned150_vs_plain1500": float(f"{np.mean([rl(corr[i], plain[i]) for i in range(ns)]):.4e}"),
+        "fallback_slices_of_first16": int((~used).sum()),
+        "threshold": a.learned_threshold,
+    }
+    del outs, plain, sparse, corr
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -326,6 +436,17 @@ def main():
     from projects_amd import gridrec as gr
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.workload == "learned":
+        b = bench_learned(a, gr, torch)
+        inf = b["inference_2048"]
+        line = {"metric": METRIC, "value": inf["learned150_slices_s"], "unit": "slices/s", "n_gpus": 1, "steps": 1,
+                "warmup": 1, "ms_per_step": round(inf["gridrec150_ms"] + inf["cnn_ms"], 2), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "bf16 network / f32 gridrec",
+                "data": "synthetic: seeded ellipse + texture phantoms, analytic Radon, noise-free",
+                "config": {"workload": "learned sparse-angle path: 2048^2 from 150 angles + 5-layer residual CNN"},
+                "learned": b}
+        print(json.dumps(line), flush=True)
+        return
     if a.workload == "precision":
         b = bench_precision(a, gr, O, torch)
         comp = b["variants"]["compensated/reference"]
@@ -366,9 +487,11 @@ def main():
     out = torch.empty((ns, out_n, out_n), dtype=torch.float32, device=dev)
     stream = torch.cuda.Stream()  # a stream of its own: events and launches on one queue
 
+    progress(f"plain: {ns} slices generated; warm-up")
     for _ in range(a.warmup):
         plan.recon(sino, out)
     torch.cuda.synchronize()
+    progress("plain: timed steps")
     plan.set_timing(True)
     plan.timing(reset=True)
     torch.cuda.synchronize()
@@ -429,6 +552,7 @@ def main():
     }
 
     # accuracy on pair 0 (the timed output)
+    progress(f"plain: {slices_s:.0f} slices/s; accuracy and CPU baseline")
     if not a.no_accuracy:
         s = sino[0:2].cpu().numpy()
         o = out[0:2].cpu().numpy()
@@ -472,6 +596,8 @@ def main():
         for v in pr["variants"].values():
             v.pop("per_slice_rel_l2_disk")  # keep the headline line short
         result["precision"] = pr
+        torch.cuda.empty_cache()
+        result["learned"] = bench_learned(a, gr, torch)
     print(json.dumps(result), flush=True)
 
 

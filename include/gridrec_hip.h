@@ -169,12 +169,15 @@ int gr_get_timing(gr_plan *plan, gr_timing *out, int reset);
  * n_slices slices [slice0, slice0+n_slices) of seed, each n_ang x n_det floats,
  * n_ell ellipses; photons > 0 adds Poisson noise and -log; density 0 is
  * U[-0.3,1], 1 is log-uniform over 4 decades; gauss_rel > 0 multiplies each
- * value by (1 + gauss_rel * z), z ~ N(0,1).  centers[s] is the axis of slice
- * s (n_slices floats, host). */
+ * value by (1 + gauss_rel * z), z ~ N(0,1); n_tex > 0 adds a band-limited
+ * random texture inside the unit disk, n_tex plane waves with frequencies
+ * uniform over |f| <= tex_fmax cycles per unit length and Gaussian amplitudes
+ * of rms tex_amp in total.  centers[s] is the axis of slice s (n_slices
+ * floats, host).  Synchronous. */
 int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0,
                          unsigned long long seed, int n_ang, int n_det,
                          const float *centers, int n_ell, float photons, int density,
-                         float gauss_rel, void *stream);
+                         float gauss_rel, int n_tex, float tex_amp, float tex_fmax, void *stream);
 
 /* Batched reconstruction of many datasets of uneven geometry, all resident in
  * device memory -- the dataset loop of the reference's drivers

@@ -540,16 +540,17 @@ int gr_get_timing(gr_plan *pl, gr_timing *out, int reset) {
 
 int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slice0, unsigned long long seed,
                          int n_ang, int n_det, const float *centers, int n_ell, float photons, int density,
-                         float gauss_rel, void *stream) {
-    if (!sino || !centers || n_slices < 1 || n_ang < 1 || n_det < 1 || n_ell < 1 || n_ell > 64)
-        return fail(GR_EINVAL, "invalid phantom arguments (1 <= n_ell <= 64)");
+                         float gauss_rel, int n_tex, float tex_amp, float tex_fmax, void *stream) {
+    if (!sino || !centers || n_slices < 1 || n_ang < 1 || n_det < 1 || n_ell < 1 || n_ell > 64 || n_tex < 0 ||
+        n_tex > 256)
+        return fail(GR_EINVAL, "invalid phantom arguments (1 <= n_ell <= 64, 0 <= n_tex <= 256)");
     if (slice_stride < (long)n_ang * n_det) return fail(GR_EINVAL, "slice stride too small");
     hipStream_t s = (hipStream_t)stream;
     float *dc = nullptr;
     GR_HIP(hipMallocAsync((void **)&dc, sizeof(float) * n_slices, s));
     GR_HIP(hipMemcpyAsync(dc, centers, sizeof(float) * n_slices, hipMemcpyHostToDevice, s));
     hipError_t e = gr_launch_phantom(sino, slice_stride, n_slices, slice0, seed, n_ang, n_det, dc, n_ell, photons,
-                                     density, gauss_rel, s);
+                                     density, gauss_rel, n_tex, tex_amp, tex_fmax, s);
     hipError_t f = hipFreeAsync(dc, s);
     hipError_t w = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "phantom kernels");

@@ -2,7 +2,8 @@
 //
 // Stockham autosort formulation (N. K. Govindaraju et al., "High performance
 // discrete Fourier transforms on graphics processors", SC'08): passes of
-// radix 4, then one radix-2 pass when log2 N is odd; output in natural order.
+// radix 16 (a 4 x 4 decomposition in registers) while 16 divides what is left,
+// then radix 4, then one radix-2 pass when needed; output in natural order.
 // Each pass reads all of a thread's butterfly inputs into registers, crosses a
 // barrier, and writes the outputs back into the same LDS row, so one buffer of
 // N complex values suffices.  SIGN = +1 computes sum_k x_k e^{+2 pi i jk/N},
@@ -35,9 +36,65 @@ __device__ __forceinline__ void gr_dft4(float2 &v0, float2 &v1, float2 &v2, floa
     v3 = make_float2(a1.x - a3.x, a1.y - a3.y);
 }
 
+// 16-point DFT in place with root w = e^{SIGN 2 pi i/16}: slots r = 4a + b
+// hold x[r]; a 4-point DFT over a for each b, the twiddle w^(bc), a 4-point
+// DFT over b for each c; X[c + 4d] ends in slot 4c + d.
+template <int SIGN>
+__device__ __forceinline__ void gr_dft16(float2 (&v)[16]) {
+    constexpr float C8 = 0.923879532511286756f, S8 = 0.382683432365089772f, R2 = 0.707106781186547524f;
+    const float s = (float)SIGN;
+#pragma unroll
+    for (int b = 0; b < 4; b++) gr_dft4<SIGN>(v[b], v[4 + b], v[8 + b], v[12 + b]);
+    v[5] = gr_cmul(v[5], make_float2(C8, s * S8));      // w^1
+    v[6] = gr_cmul(v[6], make_float2(R2, s * R2));      // w^2
+    v[7] = gr_cmul(v[7], make_float2(S8, s * C8));      // w^3
+    v[9] = gr_cmul(v[9], make_float2(R2, s * R2));      // w^2
+    v[10] = make_float2(-s * v[10].y, s * v[10].x);     // w^4 = SIGN i
+    v[11] = gr_cmul(v[11], make_float2(-R2, s * R2));   // w^6
+    v[13] = gr_cmul(v[13], make_float2(S8, s * C8));    // w^3
+    v[14] = gr_cmul(v[14], make_float2(-R2, s * R2));   // w^6
+    v[15] = gr_cmul(v[15], make_float2(-C8, -s * S8));  // w^9
+#pragma unroll
+    for (int c = 0; c < 4; c++) gr_dft4<SIGN>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
+}
+
 template <int SIGN, int MAXB>
 __device__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr) {
     int Ns = 1;
+    constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
+    const int nb16 = N >> 4;
+    while (Ns * 16 <= N) {
+        float2 v[MAXB16][16];
+        const int stride = N / (Ns * 16);
+#pragma unroll
+        for (int c = 0; c < MAXB16; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb16) {
+                const int k = j & (Ns - 1);
+#pragma unroll
+                for (int r = 0; r < 16; r++) v[c][r] = buf[j + r * nb16];
+                if (Ns > 1) {
+                    const int m = k * stride;
+#pragma unroll
+                    for (int r = 1; r < 16; r++) v[c][r] = gr_cmul(v[c][r], tw[r * m]);
+                }
+                gr_dft16<SIGN>(v[c]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < MAXB16; c++) {
+            const int j = tid + c * nthr;
+            if (j < nb16) {
+                const int k = j & (Ns - 1);
+                const int base = ((j - k) << 4) + k;
+#pragma unroll
+                for (int q = 0; q < 16; q++) buf[base + q * Ns] = v[c][4 * (q & 3) + (q >> 2)];
+            }
+        }
+        __syncthreads();
+        Ns <<= 4;
+    }
     const int nb4 = N >> 2;
     while (Ns * 4 <= N) {
         float2 v[MAXB][4];

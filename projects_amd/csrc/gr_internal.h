@@ -82,6 +82,8 @@ hipError_t gr_launch_tables_fp64(const GrGeom &g, const gr_params *p, float2 *fi
 
 // Synthetic sinograms.
 struct GrEllipse { float a, b, rho, phi, x0, y0; };
+struct GrWave { float amp, fx, fy, phase; };
 hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long slice0,
                              unsigned long long seed, int n_ang, int n_det, const float *centers_dev,
-                             int n_ell, float photons, int density, float gauss_rel, hipStream_t s);
+                             int n_ell, float photons, int density, float gauss_rel, int n_tex, float tex_amp,
+                             float tex_fmax, hipStream_t s);

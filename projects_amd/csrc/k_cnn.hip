@@ -496,14 +496,16 @@ __global__ void k_reduce(const float *__restrict__ part, int nparts, int rec, in
     }
 }
 
-// Sum of per-block partials: out[k] = sum of part[k*per + i], i < per (in order).
-__global__ void k_sum_groups(const float *__restrict__ part, int groups, int per, int stride, float *__restrict__ out) {
+// Sum of per-block partials in order, times f: out[grp*stride + comp] =
+// f * sum_i part[(grp*per + i)*stride + comp].
+__global__ void k_sum_groups(const float *__restrict__ part, int groups, int per, int stride, double f,
+                             float *__restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= groups * stride) return;
     const int grp = k / stride, comp = k - grp * stride;
     double s = 0.0;
     for (int i = 0; i < per; i++) s += (double)part[((long)grp * per + i) * stride + comp];
-    out[k] = (float)s;
+    out[k] = (float)(s * f);
 }
 
 // Packed bf16 weights of the three 32->32 layers: forward [t][co][ci] and
@@ -606,7 +608,8 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     hipLaunchKernelGGL(k_conv_out, dim3(ntiles), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W, tm);
     if (slice_stats) {
         const int per = tm.tiles_x * tm.tiles_y;
-        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, per, 2, slice_stats);
+        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, per, 2, 1.0,
+                           slice_stats);
     }
     CNN_HIP(hipGetLastError());
     return GR_OK;
@@ -642,7 +645,7 @@ int gr_cnn_backward(const float *params, const unsigned short *packed, const flo
     const int nloss = (int)std::min<long>(1024, (npx + 255) / 256);
 
     hipLaunchKernelGGL(k_loss, dim3(nloss), dim3(256), 0, s, yn, target, scale, dy, lpart, npx);
-    if (loss) hipLaunchKernelGGL(k_sum_groups, dim3(1), dim3(64), 0, s, lpart, 1, nloss, 1, loss);
+    if (loss) hipLaunchKernelGGL(k_sum_groups, dim3(1), dim3(64), 0, s, lpart, 1, nloss, 1, 1.0 / (double)npx, loss);
     // layer 5
     hipLaunchKernelGGL(k_conv_out_wgrad, dim3(nwg), dim3(320), 0, s, dy, a4, part, H, W, ntiles, tm);
     hipLaunchKernelGGL(k_reduce, dim3((OUT_REC + 255) / 256), dim3(256), 0, s, part, nwg, OUT_REC, 1,

@@ -1,5 +1,6 @@
 // k_phantom.hip -- synthetic sinograms on the device: seeded random-ellipse
-// phantoms, analytic Radon transform, Poisson photon noise and -log.
+// phantoms plus an optional band-limited random texture, analytic Radon
+// transform, Poisson photon noise and -log.
 //
 // The same generator as projects_amd/phantom.py (which documents the counter
 // layout): Philox4x32-10 counters give identical ellipse parameters and noise
@@ -101,21 +102,46 @@ __device__ float poisson_draw(float lam, unsigned k, unsigned n, unsigned sl, un
     return rintf(lam);
 }
 
+// Texture term k of a slice: amplitude, frequency (cycles per unit length)
+// and phase of amp * cos(2 pi f.r + phase) inside the unit disk.
+__global__ void k_texture(GrWave *T, int n_slices, long slice0, int n_tex, float amp, float fmax, unsigned k0,
+                          unsigned k1) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.y;
+    if (e >= n_tex || s >= n_slices) return;
+    const unsigned sl = (unsigned)(slice0 + s);
+    const Philox r = philox4x32_10((unsigned)e, sl, 3u, 0u, k0, k1);
+    const float fr = fmax * sqrtf(u01(r.x[0]));  // uniform over the disk |f| <= fmax
+    float sd, cd;
+    sincospif(2.0f * u01(r.x[1]), &sd, &cd);
+    GrWave w;
+    w.fx = fr * cd;
+    w.fy = fr * sd;
+    w.phase = 6.28318531f * u01(r.x[2]);
+    const Philox q = philox4x32_10((unsigned)e, sl, 4u, 0u, k0, k1);
+    const float z = sqrtf(-2.0f * logf(u01(q.x[0]))) * cospif(2.0f * u01(q.x[1]));
+    w.amp = amp * z / sqrtf((float)n_tex);
+    T[(long)s * n_tex + e] = w;
+}
+
 struct SinoArgs {
     float *sino;
     const GrEllipse *E;
+    const GrWave *T;
     const float *centers;
     long slice_stride, slice0;
-    int n_ang, n_det, n_ell;
+    int n_ang, n_det, n_ell, n_tex;
     float photons, gauss_rel;
     unsigned k0, k1;
 };
 
 __global__ void __launch_bounds__(256) k_sinogram(SinoArgs A) {
     __shared__ GrEllipse se[64];
+    __shared__ GrWave sw[256];
     const int s = blockIdx.z, n = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (threadIdx.x < A.n_ell) se[threadIdx.x] = A.E[(long)s * A.n_ell + threadIdx.x];
+    if (threadIdx.x < A.n_tex) sw[threadIdx.x] = A.T[(long)s * A.n_tex + threadIdx.x];
     __syncthreads();
     if (k >= A.n_det) return;
     float st, ct;
@@ -132,6 +158,19 @@ __global__ void __launch_bounds__(256) k_sinogram(SinoArgs A) {
         const float tp = t - (el.x0 * ct + el.y0 * st);
         const float d = s2 - tp * tp;
         if (d > 0.0f) p += 2.0f * el.rho * el.a * el.b * sqrtf(d) / s2;
+    }
+    // texture: the chord of the unit disk at offset t has half-length h; along
+    // it cos(alpha + beta s) integrates to 2 cos(alpha) sin(beta h) / beta
+    if (A.n_tex > 0 && fabsf(t) < 1.0f) {
+        const float h = sqrtf(1.0f - t * t);
+        for (int e = 0; e < A.n_tex; e++) {
+            const GrWave w = sw[e];
+            const float alpha = 6.28318531f * t * (w.fx * ct + w.fy * st) + w.phase;
+            const float beta = 6.28318531f * (w.fy * ct - w.fx * st);
+            const float bh = beta * h;
+            const float sinc = fabsf(bh) > 1e-4f ? sinf(bh) / bh : 1.0f - bh * bh / 6.0f;
+            p += w.amp * 2.0f * h * cosf(alpha) * sinc;
+        }
     }
     if (A.photons > 0.0f) {
         const float I = poisson_draw(A.photons * expf(-p), (unsigned)k, (unsigned)n,
@@ -150,25 +189,38 @@ __global__ void __launch_bounds__(256) k_sinogram(SinoArgs A) {
 
 hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long slice0,
                              unsigned long long seed, int n_ang, int n_det, const float *centers_dev,
-                             int n_ell, float photons, int density, float gauss_rel, hipStream_t s) {
-    if (n_ell < 1 || n_ell > 64 || n_slices < 1 || n_ang < 1 || n_det < 1) return hipErrorInvalidValue;
+                             int n_ell, float photons, int density, float gauss_rel, int n_tex, float tex_amp,
+                             float tex_fmax, hipStream_t s) {
+    if (n_ell < 1 || n_ell > 64 || n_tex < 0 || n_tex > 256 || n_slices < 1 || n_ang < 1 || n_det < 1)
+        return hipErrorInvalidValue;
     GrEllipse *E = nullptr;
+    GrWave *T = nullptr;
     hipError_t e = hipMallocAsync((void **)&E, sizeof(GrEllipse) * (size_t)n_slices * n_ell, s);
     if (e != hipSuccess) return e;
+    e = hipMallocAsync((void **)&T, sizeof(GrWave) * (size_t)n_slices * (n_tex ? n_tex : 1), s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(E, s);
+        return e;
+    }
     const unsigned k0 = (unsigned)(seed & 0xffffffffu), k1 = (unsigned)(seed >> 32);
     hipLaunchKernelGGL(k_ellipses, dim3((n_ell + 63) / 64, n_slices), dim3(64), 0, s, E, n_slices, slice0,
                        n_ell, k0, k1, density);
-    SinoArgs A{sino, E, centers_dev, slice_stride, slice0, n_ang, n_det, n_ell, photons, gauss_rel, k0, k1};
+    if (n_tex > 0)
+        hipLaunchKernelGGL(k_texture, dim3((n_tex + 63) / 64, n_slices), dim3(64), 0, s, T, n_slices, slice0, n_tex,
+                           tex_amp, tex_fmax, k0, k1);
+    SinoArgs A{sino, E, T, centers_dev, slice_stride, slice0, n_ang, n_det, n_ell, n_tex, photons, gauss_rel, k0, k1};
     for (int s0 = 0; s0 < n_slices; s0 += 65535) {
         const int ns = n_slices - s0 < 65535 ? n_slices - s0 : 65535;
         SinoArgs B = A;
         B.sino = sino + (long)s0 * slice_stride;
         B.E = E + (long)s0 * n_ell;
+        B.T = T + (long)s0 * n_tex;
         B.centers = centers_dev + s0;
         B.slice0 = slice0 + s0;
         hipLaunchKernelGGL(k_sinogram, dim3((n_det + 255) / 256, n_ang, ns), dim3(256), 0, s, B);
     }
     e = hipGetLastError();
     hipError_t f = hipFreeAsync(E, s);
-    return e != hipSuccess ? e : f;
+    hipError_t f2 = hipFreeAsync(T, s);
+    return e != hipSuccess ? e : (f != hipSuccess ? f : f2);
 }

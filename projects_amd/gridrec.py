@@ -84,7 +84,8 @@ _SIGS = {
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
     "gr_phantom_sinograms": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_int,
-                                       C.POINTER(C.c_float), C.c_int, C.c_float, C.c_int, C.c_float, C.c_void_p]),
+                                       C.POINTER(C.c_float), C.c_int, C.c_float, C.c_int, C.c_float, C.c_int,
+                                       C.c_float, C.c_float, C.c_void_p]),
     "gr_queue_create": (C.c_int, [C.POINTER(GrDataset), C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.POINTER(C.c_void_p)]),
     "gr_queue_run": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -320,7 +321,8 @@ class Queue:
 
 
 def phantom_sinograms(out, seed: int, n_ang: int, n_det: int, centers, n_ell: int, photons: float = 0.0,
-                      density: int = 0, slice0: int = 0, gauss_rel: float = 0.0, stream=None):
+                      density: int = 0, slice0: int = 0, gauss_rel: float = 0.0, n_tex: int = 0,
+                      tex_amp: float = 0.0, tex_fmax: float = 0.0, stream=None):
     """Fill the cuda tensor out[n_slices, n_ang, n_det] with the generator of
     projects_amd.phantom (slice s of the tensor is slice slice0+s of the seed)."""
     import torch
@@ -330,6 +332,6 @@ def phantom_sinograms(out, seed: int, n_ang: int, n_det: int, centers, n_ell: in
     c = np.ascontiguousarray(np.broadcast_to(np.asarray(centers, np.float32), (ns,)))
     rc = lib().gr_phantom_sinograms(C.c_void_p(out.data_ptr()), n_ang * n_det, ns, slice0, seed, n_ang, n_det,
                                     c.ctypes.data_as(C.POINTER(C.c_float)), n_ell, photons, density, gauss_rel,
-                                    C.c_void_p(_stream_handle(stream)))
+                                    n_tex, tex_amp, tex_fmax, C.c_void_p(_stream_handle(stream)))
     _check(rc, "gr_phantom_sinograms")
     return out

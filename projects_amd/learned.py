@@ -147,10 +147,11 @@ class Corrector:
         Returns (y, used) with used[i] True where the network's output was kept."""
         torch = self.torch
         y, stats = self.forward(x, out=out, stream=stream)
-        rel = torch.sqrt(stats[:, 0] / torch.clamp(stats[:, 1], min=1e-30))
-        used = rel <= self.threshold
-        if not bool(used.all()):
-            y[~used] = x[~used]
+        s = torch.cuda.current_stream() if stream is None else stream
+        with torch.cuda.stream(s):  # device-side selection, no host synchronisation
+            rel = torch.sqrt(stats[:, 0] / torch.clamp(stats[:, 1], min=1e-30))
+            used = rel <= self.threshold
+            torch.where(used.view(-1, 1, 1), y, x, out=y)
         return y, used
 
     # -- persistence --------------------------------------------------------------
@@ -174,3 +175,82 @@ class Corrector:
         c.step = int(meta["step"])
         c.meta = meta
         return c
+
+
+# -- data and training ------------------------------------------------------------
+
+def gridrec_images(sino, n_det: int, center: float, stream=None):
+    """Plain gridrec of [n, n_ang, n_det] sinograms under the half-padding caller
+    convention: [n, n_det, n_det] images (the detector's field of view)."""
+    import torch
+
+    n, n_ang, _ = sino.shape
+    M0 = (1 << (2 * n_det - 1).bit_length()) - 1
+    cn = int(round(n_det * M0 / (2 * n_det)))
+    plan = gr.Plan(n_ang=n_ang, n_det=n_det, center=center, pad="half", crop=((M0 - cn + 1) // 2, cn))
+    out = torch.empty((n, cn, cn), dtype=torch.float32, device=sino.device)
+    plan.recon(sino, out, stream=stream)
+    torch.cuda.synchronize()
+    plan.close()
+    return out
+
+
+def make_pairs(n: int, size: int = 512, sparse: int = 90, dense: int = 900, seed: int = 1, n_ell: int = 20,
+               n_tex: int = 64, tex_amp: float = 0.15, tex_fmax: float = 24.0, slice0: int = 0, chunk: int = 512):
+    """n training pairs of size x size gridrec images of the same seeded
+    phantoms (ellipses + texture, noise-free): sparse-angle input and
+    dense-angle target, both on the GPU."""
+    import torch
+
+    inp = torch.empty((n, size, size), dtype=torch.float32, device="cuda")
+    tgt = torch.empty_like(inp)
+    c = size / 2.0
+    for s0 in range(0, n, chunk):
+        k = min(chunk, n - s0)
+        for n_ang, dst in ((sparse, inp), (dense, tgt)):
+            sino = torch.empty((k, n_ang, size), dtype=torch.float32, device="cuda")
+            gr.phantom_sinograms(sino, seed, n_ang, size, [c] * k, n_ell, slice0=slice0 + s0, n_tex=n_tex,
+                                 tex_amp=tex_amp, tex_fmax=tex_fmax)
+            dst[s0:s0 + k] = gridrec_images(sino, size, c)
+            del sino
+    torch.cuda.synchronize()
+    return inp, tgt
+
+
+def train(net: Corrector, inp, tgt, steps: int = 0, seconds: float = 0.0, batch: int = 32, lr: float = 1e-3,
+          seed: int = 0, log_every: int = 50, progress=None):
+    """Adam steps on random batches of (inp, tgt); stops after `steps` steps or
+    `seconds` of wall time, whichever is set (both: the first reached).
+    Returns the loss history [(step, loss)] sampled every log_every steps."""
+    import time
+
+    import torch
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    n = inp.shape[0]
+    hist = []
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        idx = torch.randint(0, n, (batch,), device="cuda", generator=g)
+        xb, tb = inp.index_select(0, idx).contiguous(), tgt.index_select(0, idx).contiguous()
+        loss = net.train_step(xb, tb, lr=lr)
+        k += 1
+        if k % log_every == 0 or k == 1:
+            hist.append((k, float(loss.cpu())))
+            if progress and k % (log_every * 10) == 0:
+                progress(f"step {k}, loss {hist[-1][1]:.4e}")
+        if (steps and k >= steps) or (seconds and time.perf_counter() - t0 >= seconds):
+            break
+    torch.cuda.synchronize()
+    return hist, k, time.perf_counter() - t0
+
+
+def psnr(img, ref) -> float:
+    """PSNR (dB) of img against ref, peak = the dynamic range of ref."""
+    import torch
+
+    mse = torch.mean((img.double() - ref.double()) ** 2)
+    rng = ref.max().double() - ref.min().double()
+    return float(10.0 * torch.log10(rng * rng / mse))

@@ -14,6 +14,8 @@ Counter layout (c0, c1, c2, c3), key = (seed low 32 bits, seed high 32 bits):
 
 * ellipse e of slice s:  (e, s, 0, 0) -> a, b, density, rotation;
                          (e, s, 1, 0) -> centre radius, centre angle
+* texture wave e of slice s: (e, s, 3, 0) -> |f|, direction, phase;
+                             (e, s, 4, 0) -> Gaussian amplitude (Box-Muller)
 * noise of detector k, angle n, slice s, round r: (k, n, s, 2 + r)
 
 Geometry: the unit disk maps to n_det/2 detector pixels around ``center``;
@@ -22,6 +24,13 @@ ellipse with semi-axes (a, b), rotation phi, centre (x0, y0), density rho is
 p(t, theta) = 2 rho a b sqrt(s^2 - t'^2) / s^2 for |t'| < s, with
 s^2 = a^2 cos^2(theta-phi) + b^2 sin^2(theta-phi) and
 t' = t - (x0 cos theta + y0 sin theta).
+
+Texture (a band-limited random field inside the unit disk): a sum of n_tex
+plane waves amp_e cos(2 pi f_e.r + phase_e), frequencies uniform over the disk
+|f| <= fmax (cycles per unit length), amplitudes N(0, tex_amp^2 / n_tex).
+Along the chord at offset t (half-length h = sqrt(1 - t^2)) a wave integrates
+to 2 amp cos(alpha) sin(beta h) / beta with alpha = 2 pi t f.u + phase,
+beta = 2 pi f.u_perp, u = (cos theta, sin theta).
 """
 from __future__ import annotations
 
@@ -119,6 +128,46 @@ def radon_ellipses(E: Ellipses, n_ang: int, n_det: int, center: float) -> np.nda
         d = s2 - tp * tp
         p += np.where(d > 0, 2.0 * rho * a * b * np.sqrt(np.maximum(d, 0.0)) / s2, 0.0)
     return p
+
+
+@dataclass
+class Waves:
+    amp: np.ndarray
+    fx: np.ndarray
+    fy: np.ndarray
+    phase: np.ndarray
+
+
+def waves(seed: int, slice_idx: int, n_tex: int, amp: float, fmax: float) -> Waves:
+    """Texture waves of one slice (float32 values)."""
+    k0, k1 = seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF
+    e = np.arange(n_tex, dtype=np.uint32)
+    r = [u01(x) for x in philox4x32(e, slice_idx, 3, 0, k0, k1)]
+    q = [u01(x) for x in philox4x32(e, slice_idx, 4, 0, k0, k1)]
+    f = np.float32
+    fr = fmax * np.sqrt(r[0])
+    d = 2.0 * math.pi * r[1]
+    z = np.sqrt(-2.0 * np.log(q[0])) * np.cos(2.0 * math.pi * q[1])
+    return Waves((amp * z / math.sqrt(n_tex)).astype(f), (fr * np.cos(d)).astype(f), (fr * np.sin(d)).astype(f),
+                 (2.0 * math.pi * r[2]).astype(f))
+
+
+def radon_waves(T: Waves, n_ang: int, n_det: int, center: float) -> np.ndarray:
+    """Analytic line integrals of the texture on the geom-1 grid, float64."""
+    th = np.arange(n_ang, dtype=np.float64) * (math.pi / n_ang)
+    t = (np.arange(n_det, dtype=np.float64) - float(center)) / (n_det / 2.0)
+    ct, st = np.cos(th)[:, None], np.sin(th)[:, None]
+    inside = np.abs(t) < 1.0
+    h = np.sqrt(np.where(inside, 1.0 - t * t, 0.0))[None, :]
+    p = np.zeros((n_ang, n_det))
+    for i in range(len(T.amp)):
+        fx, fy = float(T.fx[i]), float(T.fy[i])
+        alpha = 2.0 * math.pi * t[None, :] * (fx * ct + fy * st) + float(T.phase[i])
+        beta = 2.0 * math.pi * (fy * ct - fx * st)
+        bh = beta * h
+        sinc = np.where(np.abs(bh) > 1e-4, np.sin(bh) / np.where(bh == 0, 1.0, bh), 1.0 - bh * bh / 6.0)
+        p += float(T.amp[i]) * 2.0 * h * np.cos(alpha) * sinc
+    return np.where(inside[None, :], p, 0.0)
 
 
 def _ptrs_constants(lam):
@@ -223,7 +272,8 @@ def gaussian(seed: int, slice_idx: int, n_ang: int, n_det: int) -> np.ndarray:
 
 
 def sinogram(seed: int, slice_idx: int, n_ang: int, n_det: int, center: float, n_ell: int,
-             photons: float = 0.0, density: int = DENSITY_UNIFORM, gauss_rel: float = 0.0) -> np.ndarray:
+             photons: float = 0.0, density: int = DENSITY_UNIFORM, gauss_rel: float = 0.0, n_tex: int = 0,
+             tex_amp: float = 0.0, tex_fmax: float = 0.0) -> np.ndarray:
     """One float32 sinogram (n_ang x n_det).
 
     photons > 0: I ~ Poisson(photons * exp(-p)), value -log(max(I,1)/photons).
@@ -232,6 +282,8 @@ def sinogram(seed: int, slice_idx: int, n_ang: int, n_det: int, center: float, n
     """
     E = ellipses(seed, slice_idx, n_ell, density)
     p = radon_ellipses(E, n_ang, n_det, center)
+    if n_tex > 0:
+        p = p + radon_waves(waves(seed, slice_idx, n_tex, tex_amp, tex_fmax), n_ang, n_det, center)
     if photons > 0:
         I = poisson(photons * np.exp(-p), seed, slice_idx)
         p = -np.log(np.maximum(I, 1.0) / photons)
@@ -240,7 +292,8 @@ def sinogram(seed: int, slice_idx: int, n_ang: int, n_det: int, center: float, n
     return p.astype(np.float32)
 
 
-def phantom_image(seed: int, slice_idx: int, n_ell: int, n: int, density: int = DENSITY_UNIFORM) -> np.ndarray:
+def phantom_image(seed: int, slice_idx: int, n_ell: int, n: int, density: int = DENSITY_UNIFORM, n_tex: int = 0,
+                  tex_amp: float = 0.0, tex_fmax: float = 0.0) -> np.ndarray:
     """The phantom itself on an n x n grid covering [-1, 1]^2 (pixel centres),
     row index = y... returned as img[x_index][y_index] to match gridrec's
     S[j][k] with j along cos (x) and k along sin (y)."""
@@ -254,4 +307,9 @@ def phantom_image(seed: int, slice_idx: int, n_ell: int, n: int, density: int = 
         xr = dx * ca + dy * sa
         yr = -dx * sa + dy * ca
         img += np.where((xr / float(E.a[i])) ** 2 + (yr / float(E.b[i])) ** 2 <= 1.0, float(E.rho[i]), 0.0)
+    if n_tex > 0:
+        T = waves(seed, slice_idx, n_tex, tex_amp, tex_fmax)
+        tex = sum(float(T.amp[i]) * np.cos(2.0 * math.pi * (float(T.fx[i]) * X + float(T.fy[i]) * Y) + float(T.phase[i]))
+                  for i in range(n_tex))
+        img += np.where(X * X + Y * Y <= 1.0, tex, 0.0)
     return img

@@ -2,8 +2,9 @@
 
 Bars:
 * each layer, fed the GPU's own BF16 input, reproduces the oracle's BF16
-  output to within one BF16 unit in the last place (FP32 accumulation order
-  may move a value across a rounding boundary) on at most 1% of the values;
+  output except where FP32 accumulation order moves a value across a rounding
+  boundary: at most 1% of the values differ, each by at most one BF16 unit or
+  (where the 288-term sum cancels) by 1e-3 of the layer's rms;
 * the FP32 output y and every parameter gradient agree with the oracle
   end to end (the BF16 activation chain of each side) to rel-L2 1e-2, the loss
   to 1e-5;
@@ -71,8 +72,9 @@ def test_forward_layers_match_oracle():
     for l in range(4):
         e = expect[l].astype(np.float32)
         d = ulp_diff(a[l], e)
+        tol = np.maximum(np.abs(e) * 2.0 ** -7, 1e-3 * np.sqrt(np.mean(e.astype(np.float64) ** 2)))
         print(f"a{l + 1}: max ulp {d.max()}, off {np.mean(d > 0):.2e}", flush=True)
-        assert d.max() <= 1 and np.mean(d > 0) < 0.01, l
+        assert np.mean(d > 0) < 0.01 and (np.abs(a[l] - e) <= tol).all(), l
     r = K.conv3x3(a[3], wq["W5"][None])[..., 0] + wq["b5"][0]
     yn = x / np.float32(net.scale) + r
     assert rel(y.cpu().numpy(), yn * net.scale) < 1e-5
@@ -150,3 +152,21 @@ def test_fallback_keeps_plain_image():
     net.threshold = 1e9
     y, used = net.correct(xg)
     assert bool(used.all()) and not torch.equal(y, xg)
+
+
+def test_trained_network_beats_sparse_gridrec():
+    """Train on gridrec pairs (24 / 240 angles, 128^2 phantoms with texture) and
+    compare on held-out phantoms: the corrected sparse-angle images are closer
+    to the dense-angle gridrec than the plain sparse-angle ones."""
+    inp, tgt = Lrn.make_pairs(512, 128, 24, 240, seed=9, n_tex=32, tex_fmax=12.0)
+    net = Lrn.Corrector(seed=9, scale=float(tgt.std()))
+    hist, steps, _ = Lrn.train(net, inp, tgt, steps=400, batch=16, lr=2e-3)
+    vi, vt = Lrn.make_pairs(32, 128, 24, 240, seed=9, n_tex=32, tex_fmax=12.0, slice0=100000)
+    vy, used = net.correct(vi)
+    torch.cuda.synchronize()
+    plain = np.mean([Lrn.psnr(vi[i], vt[i]) for i in range(32)])
+    learned = np.mean([Lrn.psnr(vy[i], vt[i]) for i in range(32)])
+    print(f"loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}; held-out PSNR plain {plain:.2f} dB, learned {learned:.2f} dB",
+          flush=True)
+    assert bool(used.all())
+    assert learned > plain + 1.0

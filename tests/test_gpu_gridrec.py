@@ -229,6 +229,16 @@ def test_phantom_kernel_matches_numpy():
         assert np.abs(got[s] - ref).max() < 5e-3 * np.abs(ref).max(), s
 
 
+def test_phantom_kernel_texture_matches_numpy():
+    n_ang, n_det = 60, 256
+    out = torch.empty((2, n_ang, n_det), device="cuda")
+    gr.phantom_sinograms(out, 77, n_ang, n_det, [128.0, 127.4], 6, n_tex=48, tex_amp=0.2, tex_fmax=12.0)
+    got = out.cpu().numpy()
+    for s, c in enumerate([128.0, 127.4]):
+        ref = phantom.sinogram(77, s, n_ang, n_det, c, 6, n_tex=48, tex_amp=0.2, tex_fmax=12.0)
+        assert O.rel_l2(got[s], ref) < 1e-4, s
+
+
 def test_phantom_kernel_noise_statistics():
     n_ang, n_det = 64, 512
     out = torch.empty((1, n_ang, n_det), device="cuda")
