@@ -67,16 +67,17 @@ __device__ __forceinline__ void accumulate(float4 &acc, float xr, float xi) {
     }
 }
 
-// Stage samples c0 .. c0+nj-1 of angle n for the P pairs (zeros for the
-// nvalid..P-1 slots of a partial group), then add the taps of samples
-// [j0, j1] into cell (qa, qb).
+// Stage samples c0 .. c0+nj-1 of angle n for the P pairs, then add the taps
+// of samples [j0, j1] into cell (qa, qb).  In a partial group (nvalid < P)
+// the spare slots repeat the last valid pair: their accumulators are never
+// stored, and unconditional loads keep the P loads issued back to back.
 template <int ACCUM, int P>
 __device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int nvalid, long row_off, int c0, int nj, int j0,
                                       int j1, float C, float S, float qa, float qb, const GrGatherGeom &g,
                                       const float *wtbl, float2 (*stage)[64], int lane, float4 (&acc)[P]) {
 #pragma unroll
     for (int p = 0; p < P; p++)
-        if (lane < nj) stage[p][lane] = p < nvalid ? spec[p * pair_stride + row_off + c0 + lane] : make_float2(0.0f, 0.0f);
+        if (lane < nj) stage[p][lane] = spec[(p < nvalid ? p : nvalid - 1) * pair_stride + row_off + c0 + lane];
     __builtin_amdgcn_wave_barrier();
     for (int j = j0; j <= j1; j++) {
         float w;
