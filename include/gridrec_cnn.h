@@ -52,6 +52,14 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
                    float scale, unsigned short *a1, unsigned short *a2, unsigned short *a3,
                    unsigned short *a4, float *y, float *yn, float *slice_stats, void *work, void *stream);
 
+/* Inference: gr_cnn_forward's y (bitwise) with the five layers fused in one
+ * kernel and no activation buffers; slice_stats as there (values may differ
+ * in the last bits: another summation order).  work: gr_cnn_infer_work_bytes(n,
+ * H, W) bytes when slice_stats is given, else may be NULL. */
+size_t gr_cnn_infer_work_bytes(int n, int H, int W);
+int gr_cnn_infer(const float *params, const unsigned short *packed, const float *x, int n, int H, int W,
+                 float scale, float *y, float *slice_stats, void *work, void *stream);
+
 /* Backward pass of the MSE loss mean((yn - target/scale)^2): the gradient of
  * every parameter into grads (param layout, overwritten) and the loss into
  * *loss (device float, may be NULL).  a1..a4 and yn from gr_cnn_forward on

@@ -159,7 +159,7 @@ __device__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, in
 }
 
 // Threads per FFT block and the matching MAXB for a row of N points.
-static inline int gr_fft_threads(long N) { return N >= 1024 ? 256 : (N >= 256 ? (int)(N / 4) : 64); }
+static inline int gr_fft_threads(long N) { return N >= 8192 ? 512 : (N >= 1024 ? 256 : (N >= 256 ? (int)(N / 4) : 64)); }
 
 // Allow a kernel more than 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
 template <typename K>

@@ -57,6 +57,7 @@ constexpr int WG_REC = 9 * CH * CH + CH;     // weight-gradient partial record o
 constexpr int IN_REC = 9 * CH + CH;          // of the 1->32 layer
 constexpr int OUT_REC = 9 * CH + 1;          // of the 32->1 layer
 constexpr int MAX_WG_BLOCKS = 512;
+constexpr int MAX_CONV_BLOCKS = 2048;  // 8 per CU: each loops over tiles with its weights in registers
 
 static_assert(N_PARAMS == 28353, "parameter layout");
 
@@ -68,6 +69,22 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
 __device__ __forceinline__ bool positive_bf16(uint16_t h) { return !(h & 0x8000u) && h != 0; }
+
+// s + sum over ci = 0..31 (in order) of w[ci*9 + t] * px[ci], px one pixel's
+// 32 bf16 channels in LDS (16-byte aligned), read 8 channels at a time.
+__device__ __forceinline__ float dot_tap(const uint16_t *px, const float *w, int t, float s) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(px + 8 * q);
+        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const float a = __uint_as_float(e & 1 ? (u[e >> 1] & 0xffff0000u) : (u[e >> 1] << 16));
+            s += w[(8 * q + e) * 9 + t] * a;
+        }
+    }
+    return s;
+}
 
 struct TileMap {
     int tiles_x, tiles_y;
@@ -86,12 +103,22 @@ struct TileMap {
 template <int MODE>
 __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, const uint16_t *__restrict__ wp,
                                                 const float *__restrict__ bias, const uint16_t *__restrict__ mask,
-                                                uint16_t *__restrict__ y, int H, int W, TileMap tm) {
+                                                uint16_t *__restrict__ y, int H, int W, int ntiles, TileMap tm) {
     __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
-    int img, y0, x0;
-    tm.of(blockIdx.x, img, y0, x0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // weight fragments, loaded once per wave for all its tiles: lane holds
+    // row co = nb*16 + (lane & 15), k = ci = 8*(lane >> 4) + e
+    bf16x8 A[2][9];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    int img, y0, x0;
+    tm.of(tile, img, y0, x0);
     const long ibase = (long)img * H * W;
+    __syncthreads();  // the previous tile's reads of sx are done
     for (int i = tid; i < HS * WS * 4; i += 256) {
         const int pix = i >> 2, ch = i & 3;
         const int r = pix / WS, c = pix - r * WS;
@@ -101,13 +128,6 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
             v = *reinterpret_cast<const uint4 *>(x + ((ibase + (long)gy * W + gx) * CH + ch * 8));
         *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
     }
-    // weight fragments: lane holds row co = nb*16 + (lane & 15), k = ci = 8*(lane >> 4) + e
-    bf16x8 A[2][9];
-#pragma unroll
-    for (int nb = 0; nb < 2; nb++)
-#pragma unroll
-        for (int t = 0; t < 9; t++)
-            A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
     __syncthreads();
 
     const int j = lane & 15, g = lane >> 4;
@@ -151,6 +171,7 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
             }
         }
     }
+  }
 }
 
 // Weight gradient of a 32 -> 32 layer: partial[block] = (dW[t][co][ci], db[co])
@@ -347,10 +368,8 @@ __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a
     float rr = 0.0f, xx = 0.0f;
     if (gy < H && gx < W) {
         float s = 0.0f;
-        for (int t = 0; t < 9; t++) {
-            const uint16_t *px = sx + ((r + t / 3) * WS + c + t % 3) * PSTR;
-            for (int ci = 0; ci < CH; ci++) s += sw[ci * 9 + t] * from_bf16(px[ci]);
-        }
+        for (int t = 0; t < 9; t++)
+            s = dot_tap(sx + ((r + t / 3) * WS + c + t % 3) * PSTR, sw, t, s);
         s += sw[9 * CH];
         const long p = ibase + (long)gy * W + gx;
         const float xn = x[p] / scale;
@@ -373,6 +392,159 @@ __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a
     if (tid == 0 && stats) {
         stats[2 * blockIdx.x] = sred[0][0];
         stats[2 * blockIdx.x + 1] = sred[1][0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inference, all five layers fused: a block owns an FH x FW output tile and
+// computes each layer on the tile grown by the halo the later layers need
+// (x: 5 pixels, a1: 4, a2: 3, a3: 2, a4: 1), keeping every activation in LDS
+// (a1/a3 and a2/a4 share two buffers), so HBM sees x once and y once instead
+// of four 64-byte activation records per pixel each way.  Per pixel the
+// arithmetic and its order are those of k_conv_in, k_conv32<0> and
+// k_conv_out, so y is bitwise the unfused forward's; activations of pixels
+// outside the image are zero, the unfused path's zero padding.
+constexpr int FH = 16, FW = 32;
+constexpr int XR = FH + 10, XC = FW + 10;
+constexpr int R1 = FH + 8, C1 = FW + 8;
+constexpr int R2 = FH + 6, C2 = FW + 6;
+constexpr size_t FUSED_LDS = sizeof(float) * XR * XC + sizeof(uint16_t) * (size_t)(R1 * C1 + R2 * C2) * PSTR +
+                             sizeof(float) * (IN_REC + OUT_REC) + sizeof(float) * 2 * FH * FW;
+
+// One 32 -> 32 layer from an (Ri x Ci) input region to the (Ri-2 x Ci-2)
+// output region, LDS to LDS; (oy, ox) is the image position of output (0, 0).
+__device__ __forceinline__ void fused_conv32(const uint16_t *in, int Ci, uint16_t *out, int Ro, int Co,
+                                             const uint16_t *__restrict__ wp, const float *__restrict__ bias,
+                                             int oy, int ox, int H, int W, int lane, int wave, int nwaves) {
+    bf16x8 A[2][9];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
+    const int j = lane & 15, g = lane >> 4;
+    const int npix = Ro * Co, ngroups = (npix + 15) / 16;
+    for (int grp = wave; grp < ngroups; grp += nwaves) {
+        const int p = grp * 16 + j;
+        const int pc = p < npix ? p : npix - 1;
+        const int r = pc / Co, c = pc - r * Co;
+        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 9; t++) {
+            const int ky = t / 3, kx = t - ky * 3;
+            const bf16x8 B = *reinterpret_cast<const bf16x8 *>(in + ((r + ky) * Ci + c + kx) * PSTR + 8 * g);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][t], B, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][t], B, acc1, 0, 0, 0);
+        }
+        if (p < npix) {
+            const int gy = oy + r, gx = ox + c;
+            const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++) {
+                const floatx4 a = nb ? acc1 : acc0;
+                const int co0 = nb * 16 + 4 * g;
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float z = a[e] + bias[co0 + e];
+                    v[e] = inside && z > 0.0f ? z : 0.0f;
+                }
+                uint2 o;
+                o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                *reinterpret_cast<uint2 *>(out + p * PSTR + co0) = o;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(512) k_cnn_fused(const float *__restrict__ x, float scale,
+                                                   const float *__restrict__ prm, const uint16_t *__restrict__ packed,
+                                                   float *__restrict__ y, float *__restrict__ stats, int H, int W,
+                                                   int tiles_x, int tiles_y) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float *sx = reinterpret_cast<float *>(smem_raw);                 // [XR][XC] bf16(x/scale) as float
+    uint16_t *bufA = reinterpret_cast<uint16_t *>(sx + XR * XC);     // a1 [R1][C1], then a3
+    uint16_t *bufB = bufA + R1 * C1 * PSTR;                          // a2 [R2][C2], then a4
+    float *sw1 = reinterpret_cast<float *>(bufB + R2 * C2 * PSTR);   // W1 (bf16 values), b1
+    float *sw5 = sw1 + IN_REC;                                       // W5 (bf16 values), b5
+    float *sred = sw5 + OUT_REC;                                     // [2][FH*FW]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int per = tiles_x * tiles_y;
+    const int img = blockIdx.x / per, rt = blockIdx.x - img * per;
+    const int y0 = (rt / tiles_x) * FH, x0 = (rt % tiles_x) * FW;
+    const long ibase = (long)img * H * W;
+
+    for (int i = tid; i < 9 * CH; i += blockDim.x) sw1[i] = round_bf16(prm[OFF_W1 + i]);
+    for (int i = tid; i < CH; i += blockDim.x) sw1[9 * CH + i] = prm[OFF_B1 + i];
+    for (int i = tid; i < 9 * CH; i += blockDim.x) sw5[i] = round_bf16(prm[OFF_W5 + i]);
+    if (tid == 0) sw5[9 * CH] = prm[OFF_B5];
+    for (int i = tid; i < XR * XC; i += blockDim.x) {
+        const int r = i / XC, c = i - r * XC, gy = y0 - 5 + r, gx = x0 - 5 + c;
+        sx[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
+    }
+    __syncthreads();
+    // layer 1 on the a1 region
+    for (int p = tid; p < R1 * C1; p += blockDim.x) {
+        const int r = p / C1, c = p - r * C1, gy = y0 - 4 + r, gx = x0 - 4 + c;
+        const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        float xv[9];
+#pragma unroll
+        for (int t = 0; t < 9; t++) xv[t] = sx[(r + t / 3) * XC + c + t % 3];
+        uint32_t packedw[16];
+#pragma unroll
+        for (int co = 0; co < CH; co++) {
+            float z = 0.0f;
+#pragma unroll
+            for (int t = 0; t < 9; t++) z += sw1[co * 9 + t] * xv[t];
+            z += sw1[9 * CH + co];
+            const uint16_t h = to_bf16(inside && z > 0.0f ? z : 0.0f);
+            if (co & 1) packedw[co >> 1] |= (uint32_t)h << 16;
+            else packedw[co >> 1] = h;
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(bufA + p * PSTR);
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            dst[k] = make_uint4(packedw[4 * k], packedw[4 * k + 1], packedw[4 * k + 2], packedw[4 * k + 3]);
+    }
+    __syncthreads();
+    fused_conv32(bufA, C1, bufB, R2, C2, packed, prm + OFF_B2, y0 - 3, x0 - 3, H, W, lane, wave, nwaves);
+    __syncthreads();
+    fused_conv32(bufB, C2, bufA, R2 - 2, C2 - 2, packed + PACK_LAYER, prm + OFF_B3, y0 - 2, x0 - 2, H, W, lane, wave,
+                 nwaves);
+    __syncthreads();
+    fused_conv32(bufA, C2 - 2, bufB, R2 - 4, C2 - 4, packed + 2 * PACK_LAYER, prm + OFF_B4, y0 - 1, x0 - 1, H, W,
+                 lane, wave, nwaves);
+    __syncthreads();
+    // layer 5 + residual, one output pixel per thread
+    const int C4 = C2 - 4;
+    float rr = 0.0f, xx = 0.0f;
+    for (int q = tid; q < FH * FW; q += blockDim.x) {
+        const int r = q / FW, c = q - r * FW, gy = y0 + r, gx = x0 + c;
+        if (gy < H && gx < W) {
+            float s = 0.0f;
+            for (int t = 0; t < 9; t++) s = dot_tap(bufB + ((r + t / 3) * C4 + c + t % 3) * PSTR, sw5, t, s);
+            s += sw5[9 * CH];
+            const long p = ibase + (long)gy * W + gx;
+            const float xn = x[p] / scale;
+            y[p] = (xn + s) * scale;
+            rr += s * s;
+            xx += xn * xn;
+        }
+    }
+    sred[tid] = rr;
+    sred[FH * FW + tid] = xx;
+    __syncthreads();
+    for (int h = FH * FW / 2; h > 0; h >>= 1) {
+        for (int i = tid; i < h; i += blockDim.x) {
+            sred[i] += sred[i + h];
+            sred[FH * FW + i] += sred[FH * FW + i + h];
+        }
+        __syncthreads();
+    }
+    if (tid == 0 && stats) {
+        stats[2 * blockIdx.x] = sred[0];
+        stats[2 * blockIdx.x + 1] = sred[FH * FW];
     }
 }
 
@@ -594,9 +766,10 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, x, scale, params, a1, n, H, W);
     uint16_t *acts[4] = {a1, a2, a3, a4};
     const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
+    const int nconv = ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS;
     for (int l = 0; l < 3; l++)
-        hipLaunchKernelGGL(k_conv32<0>, dim3(ntiles), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
-                           params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, tm);
+        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
+                           params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, ntiles, tm);
     float *stats = nullptr;
     if (slice_stats) {
         char *wb = (char *)work;
@@ -611,6 +784,32 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
         hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, per, 2, 1.0,
                            slice_stats);
     }
+    CNN_HIP(hipGetLastError());
+    return GR_OK;
+}
+
+size_t gr_cnn_infer_work_bytes(int n, int H, int W) {
+    return sizeof(float) * 2 * (size_t)((W + FW - 1) / FW) * ((H + FH - 1) / FH) * (n > 0 ? n : 0);
+}
+
+int gr_cnn_infer(const float *params, const unsigned short *packed, const float *x, int n, int H, int W, float scale,
+                 float *y, float *slice_stats, void *work, void *stream) {
+    if (!params || !packed || !x || !y || !(scale > 0.0f)) return cnn_fail(GR_EINVAL, "null argument or scale <= 0");
+    int rc = check_dims(n, H, W);
+    if (rc) return rc;
+    if (slice_stats && !work) return cnn_fail(GR_EINVAL, "slice statistics need the work buffer");
+    hipStream_t s = (hipStream_t)stream;
+    const int tx = (W + FW - 1) / FW, ty = (H + FH - 1) / FH;
+    const long nblocks = (long)tx * ty * n;
+    const long npx = (long)n * H * W;
+    (void)npx;
+    float *stats = slice_stats ? (float *)work : nullptr;
+    CNN_HIP(gr_allow_lds(k_cnn_fused, FUSED_LDS));
+    hipLaunchKernelGGL(k_cnn_fused, dim3((unsigned)nblocks), dim3(FH * FW), FUSED_LDS, s, x, scale, params,
+                       reinterpret_cast<const uint16_t *>(packed), y, stats, H, W, tx, ty);
+    if (slice_stats)
+        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, tx * ty, 2, 1.0,
+                           slice_stats);
     CNN_HIP(hipGetLastError());
     return GR_OK;
 }
@@ -662,8 +861,9 @@ int gr_cnn_backward(const float *params, const unsigned short *packed, const flo
         hipLaunchKernelGGL(k_conv32_wgrad, dim3(nwg), dim3(256), WGRAD_LDS, s, ain, dz, part, H, W, ntiles, tm);
         hipLaunchKernelGGL(k_reduce, dim3((WG_REC + 255) / 256), dim3(256), 0, s, part, nwg, WG_REC, 0,
                            grads + woff[l], grads + boff[l]);
-        hipLaunchKernelGGL(k_conv32<1>, dim3(ntiles), dim3(256), 0, s, dz, pk + (long)l * PACK_LAYER + 9 * CH * CH,
-                           (const float *)nullptr, ain, dzn, H, W, tm);
+        hipLaunchKernelGGL(k_conv32<1>, dim3(ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS), dim3(256), 0, s,
+                           dz, pk + (long)l * PACK_LAYER + 9 * CH * CH, (const float *)nullptr, ain, dzn, H, W,
+                           ntiles, tm);
         std::swap(dz, dzn);
     }
     // layer 1

@@ -31,7 +31,7 @@ struct RowsArgs {
 };
 
 template <int MAXB>
-__global__ void __launch_bounds__(256) k_ifft_rows(RowsArgs A) {
+__global__ void __launch_bounds__(512) k_ifft_rows(RowsArgs A) {
     extern __shared__ float2 row[];
     const int b = blockIdx.x, p = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -63,7 +63,7 @@ __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
 }
 
 template <int MAXB>
-__global__ void __launch_bounds__(256) k_ifft_cols(ColsArgs A) {
+__global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
     extern __shared__ float2 cols[];  // cw columns of M
     const int p = blockIdx.y;
     const long c0 = column_group(blockIdx.x, gridDim.x) * A.cw;

@@ -32,7 +32,7 @@ struct RowFftArgs {
 };
 
 template <int MAXB>
-__global__ void __launch_bounds__(256) k_rowfft(RowFftArgs A) {
+__global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     extern __shared__ float2 row[];
     const int n = blockIdx.x;
     const int p = blockIdx.y;

@@ -101,6 +101,9 @@ _SIGS = {
     "gr_cnn_pack": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "gr_cnn_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float]
                        + [C.c_void_p] * 9),
+    "gr_cnn_infer_work_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "gr_cnn_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_float]
+                     + [C.c_void_p] * 4),
     "gr_cnn_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                   C.c_float] + [C.c_void_p] * 9),
     "gr_cnn_adam": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_long, C.c_int, C.c_float,

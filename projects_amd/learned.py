@@ -82,18 +82,19 @@ class Corrector:
         self.pack()
 
     # -- buffers ----------------------------------------------------------------
-    def _buffers(self, n, H, W, train):
-        key = (n, H, W, train)
+    def _buffers(self, n, H, W, train, acts=True):
+        key = (n, H, W, train, acts)
         if key not in self._bufs:
             torch = self.torch
             for k in [k for k in self._bufs if k[3] == train]:  # keep one shape per mode
                 del self._bufs[k]
-            na = 4 if train else 2
-            acts = [torch.empty((n, H, W, CH), dtype=torch.int16, device=self.dev) for _ in range(na)]
-            work = torch.empty(gr.lib().gr_cnn_work_bytes(n, H, W), dtype=torch.uint8, device=self.dev)
+            na = 0 if not acts else (4 if train else 2)
+            acts_l = [torch.empty((n, H, W, CH), dtype=torch.int16, device=self.dev) for _ in range(na)]
+            wb = gr.lib().gr_cnn_work_bytes(n, H, W) if na else gr.lib().gr_cnn_infer_work_bytes(n, H, W)
+            work = torch.empty(wb, dtype=torch.uint8, device=self.dev)
             yn = torch.empty((n, H, W), dtype=torch.float32, device=self.dev) if train else None
             stats = torch.empty((n, 2), dtype=torch.float32, device=self.dev)
-            self._bufs[key] = (acts, work, yn, stats)
+            self._bufs[key] = (acts_l, work, yn, stats)
         return self._bufs[key]
 
     def pack(self, stream=None):
@@ -116,6 +117,21 @@ class Corrector:
                                      *[_ptr(t) for t in a], _ptr(out), _ptr(yn), _ptr(stats), _ptr(work),
                                      C.c_void_p(gr._stream_handle(stream)))
         gr._check(rc, "gr_cnn_forward")
+        return out, stats
+
+    def infer(self, x, out=None, stream=None):
+        """Inference in one fused kernel, bitwise forward()'s y, with no
+        activation buffers (for memory-tight callers; forward() is faster on
+        MI355X at 2048^2): returns (y, stats)."""
+        torch = self.torch
+        assert x.is_cuda and x.dtype == torch.float32 and x.dim() == 3 and x.is_contiguous()
+        n, H, W = x.shape
+        _, work, _, stats = self._buffers(n, H, W, False, acts=False)
+        if out is None:
+            out = torch.empty_like(x)
+        rc = gr.lib().gr_cnn_infer(_ptr(self.params), _ptr(self.packed), _ptr(x), n, H, W, self.scale, _ptr(out),
+                                   _ptr(stats), _ptr(work), C.c_void_p(gr._stream_handle(stream)))
+        gr._check(rc, "gr_cnn_infer")
         return out, stats
 
     def backward(self, x, target, stream=None):

@@ -86,6 +86,19 @@ def test_forward_layers_match_oracle():
     assert rel(y.cpu().numpy(), c["y"] * net.scale) < 1e-2
 
 
+@pytest.mark.parametrize("shape", [(3, 37, 70), (1, 16, 32), (2, 5, 7), (1, 130, 200)])
+def test_fused_inference_is_bitwise_forward(shape):
+    """The fused kernel (all layers in LDS, halo recomputation) gives bitwise
+    the layer-by-layer forward's output, borders and ragged tiles included."""
+    net, _, _, _ = make(6)
+    x = torch.randn(shape, device="cuda") * 2.0
+    y1, s1 = net.forward(x)
+    y2, s2 = net.infer(x)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    assert torch.allclose(s1, s2, rtol=1e-4)
+
+
 def test_backward_matches_oracle():
     net, pd, x, t = make(2)
     xg, tg = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
