@@ -16,6 +16,9 @@
  *   gr_recon_device  <- do_recon over a batch of slice pairs already in HBM
  *                       (the pair loop of Multitomo_unit.txt:823-856 and
  *                       tomo_recon.cpp:3700-3863 moved onto the GPU)
+ *   gr_recon_device_raw <- the same from detector counts, with the flat/dark
+ *                       normalisation and sub-pixel shift of tomo_recon.cpp:1877,
+ *                       3708-3780 (or find_shift.cpp:1685-1697) fused in
  *   gr_plan_destroy  <- recon_free        grid.cpp:418-434
  *
  * All functions return 0 on success and a negative GR_E* code on failure;
@@ -156,6 +159,29 @@ int gr_recon_host(gr_plan *plan, const float *const *G1, const float *const *G2,
 int gr_recon_device(gr_plan *plan, const float *sino, long sino_slice_stride,
                     long sino_row_stride, float *out, long out_slice_stride,
                     long out_row_stride, int n_slices, void *stream);
+
+/* do_recon from detector counts: the preprocessing the callers run on the
+ * host before gridrec, fused into the row load of the same kernels.
+ *   GR_PRE_FLATDARK:     v = 1 - (p - d) / (w1/2 + w2/2 - d)   (tomo_recon.cpp:1877)
+ *   GR_PRE_FLATDARK_LOG: v = log((w1/2 + w2/2 - d) / (p - d)), 0 where p <= d or
+ *                        the ratio is <= 0 (find_shift.cpp:1685-1697; float here)
+ * then each row is shifted by xoffset cells with linear interpolation, cells
+ * beyond the ends taking the edge value (tomo_recon.cpp:3708-3780; 0 = no
+ * shift), then padded as the plan says.  raw: counts [slice][n_ang][row_stride];
+ * dark, white1, white2: one row of n_det floats per slice, field_stride apart
+ * (white2 may be NULL: white1 alone).  Slices pair and images land as in
+ * gr_recon_device. */
+enum { GR_PRE_FLATDARK = 1, GR_PRE_FLATDARK_LOG = 2 };
+typedef struct gr_preproc {
+    int mode;
+    const unsigned short *raw;
+    long raw_slice_stride, raw_row_stride;
+    const float *dark, *white1, *white2;
+    long field_stride;
+    float xoffset;
+} gr_preproc;
+int gr_recon_device_raw(gr_plan *plan, const gr_preproc *pre, float *out, long out_slice_stride,
+                        long out_row_stride, int n_slices, void *stream);
 
 /* recon_free. */
 int gr_plan_destroy(gr_plan *plan);

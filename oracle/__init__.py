@@ -111,6 +111,7 @@ def lib():
         _lib.or_recon_mixed.argtypes = [P, fp, fp, dp, dp, C.c_int]
         _lib.or_pad_half.argtypes = [C.c_int, C.c_int, fp, fp]
         _lib.or_pad_half.restype = None
+        _lib.or_preprocess.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ushort), fp, fp, fp, C.c_float, fp]
         _lib.or_fft_f32.argtypes = [fp, C.c_long, C.c_int]
         _lib.or_fft_f64.argtypes = [dp, C.c_long, C.c_int]
         _lib.or_recon_batch_f32.argtypes = [P, C.c_int, fp, C.c_int, C.c_long, C.c_long, fp, C.c_int]
@@ -218,6 +219,22 @@ def recon_mixed(g: Geometry, g1, g2, f64_mask: int):
     if lib().or_recon_mixed(C.byref(p), _fptr(g1), _fptr(g2), _dptr(s1), _dptr(s2), int(f64_mask)):
         raise RuntimeError("or_recon_mixed failed")
     return s1, s2
+
+
+def preprocess(mode: int, raw: np.ndarray, dark, white1, white2=None, xoffset: float = 0.0) -> np.ndarray:
+    """Flat/dark normalisation (mode 1: tomo_recon.cpp:1877, mode 2: the log form
+    of find_shift.cpp:1685-1697) and sub-pixel shift of one slice's counts."""
+    raw = np.ascontiguousarray(raw, dtype=np.uint16)
+    n_ang, n_det = raw.shape
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    dark, white1 = f(dark), f(white1)
+    white2 = f(white2) if white2 is not None else None
+    out = np.empty((n_ang, n_det), np.float32)
+    rc = lib().or_preprocess(mode, n_ang, n_det, raw.ctypes.data_as(C.POINTER(C.c_ushort)), _fptr(dark), _fptr(white1),
+                             _fptr(white2) if white2 is not None else None, xoffset, _fptr(out))
+    if rc:
+        raise RuntimeError("or_preprocess failed")
+    return out
 
 
 def pad_half(rows: np.ndarray) -> np.ndarray:

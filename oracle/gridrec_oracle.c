@@ -763,6 +763,44 @@ out:
 /* ---------------------------------------------------------------------------
  * Caller conventions and the threaded batch driver.
  * ------------------------------------------------------------------------- */
+static float or_normalise(int mode, unsigned short raw, float dark, float w1, float w2) {
+    if (mode == 1) {  /* tomo_recon.cpp:1877 */
+        const float proj = (float)raw;
+        return 1.0f - 1.0f * (proj - dark) / (0.5f * w1 + 0.5f * w2 - dark);
+    }
+    /* find_shift.cpp:1685-1697: doubles, result stored as float */
+    double dv = (double)raw - (double)dark;
+    if (dv <= 0) return 0.0f;
+    dv = (0.5 * ((double)w1 + (double)w2) - (double)dark) / dv;
+    return dv > 0.0 ? (float)log(dv) : 0.0f;
+}
+
+int or_preprocess(int mode, int n_ang, int n_det, const unsigned short *raw, const float *dark,
+                  const float *white1, const float *white2, float xoffset, float *out) {
+    if (mode != 1 && mode != 2) return -1;
+    if (!white2) white2 = white1;
+    float *norm = (float *)malloc(sizeof(float) * (size_t)n_det);
+    if (!norm) return -1;
+    for (int j = 0; j < n_ang; j++) {
+        for (int k = 0; k < n_det; k++)
+            norm[k] = or_normalise(mode, raw[(size_t)j * n_det + k], dark[k], white1[k], white2[k]);
+        /* tomo_recon.cpp:3708-3780: kk = k - xoffset, two-point linear
+           interpolation, an index beyond either end reads the edge cell */
+        for (int k = 0; k < n_det; k++) {
+            const float kk = (float)k - xoffset;
+            const int nkk = (int)floorf(kk);
+            const int i0 = nkk < 0 ? 0 : (nkk >= n_det ? n_det - 1 : nkk);
+            const int i1 = nkk + 1 < 0 ? 0 : (nkk + 1 >= n_det ? n_det - 1 : nkk + 1);
+            float v = 0.0f;
+            v += norm[i0] * (float)(nkk + 1 - kk);
+            v += norm[i1] * (kk - (float)nkk);
+            out[(size_t)j * n_det + k] = v;
+        }
+    }
+    free(norm);
+    return 0;
+}
+
 void or_pad_half(int n_ang, int n_det, const float *src, float *dst) {
     int off = n_det / 2, w = 2 * n_det;                   /* tomo_recon.cpp:3692,3698 */
     for (int a = 0; a < n_ang; a++) {

@@ -91,6 +91,16 @@ int or_recon_mixed(const or_params *p, const float *g1, const float *g2,
  * dst must hold n_ang * 2*n_det floats. */
 void or_pad_half(int n_ang, int n_det, const float *src, float *dst);
 
+/* The callers' preprocessing of detector counts before gridrec, n_ang rows of
+ * n_det counts, one row each of dark, white1, white2:
+ *   mode 1 -- 1 - (p - d)/(0.5 w1 + 0.5 w2 - d) in float (tomo_recon.cpp:1877);
+ *   mode 2 -- log((0.5 (w1 + w2) - d)/(p - d)) in double, 0 where p - d <= 0 or
+ *             the ratio is <= 0 (find_shift.cpp:1685-1697);
+ * then each row is shifted by xoffset with linear interpolation and edge
+ * extension (tomo_recon.cpp:3708-3780).  Returns 0, or -1 on a bad mode. */
+int or_preprocess(int mode, int n_ang, int n_det, const unsigned short *raw, const float *dark,
+                  const float *white1, const float *white2, float xoffset, float *out);
+
 /* Batch driver used for the CPU baseline: n_pairs slice pairs of the raw
  * (unpadded) sinograms sino[2*pair + {0,1}][n_ang][n_det]; when pad_half is set
  * each pair is padded with or_pad_half and reconstructed with center + n_det/2;

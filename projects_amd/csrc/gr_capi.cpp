@@ -436,15 +436,16 @@ int gr_plan_tables(const gr_plan *pl, float *sine, float *cose, float *filphase,
     return GR_OK;
 }
 
-int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride, float *out,
-                    long out_slice_stride, long out_row_stride, int n_slices, void *stream) {
-    if (!pl || !sino || !out) return fail(GR_EINVAL, "null argument");
-    if (n_slices < 0) return fail(GR_EINVAL, "negative slice count");
-    if (n_slices == 0) return GR_OK;
+}  // extern "C"
+
+namespace {
+
+// The chunk loop of gr_recon_device / gr_recon_device_raw: K1..K4 per chunk
+// of max_pairs pairs; K1 reads float rows (pre null) or detector counts.
+int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride, const GrPre *pre,
+                 float *out, long out_slice_stride, long out_row_stride, int n_slices, void *stream) {
     if (!pl->d_spec1) return fail(GR_EINVAL, "plan has no workspace");
     const GrGeom &g = pl->g;
-    if (sino_row_stride < g.n_det_in || sino_slice_stride < sino_row_stride * (long)pl->prm.n_ang)
-        return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
     if (out_row_stride < g.out_n || out_slice_stride < out_row_stride * g.out_n)
         return fail(GR_EINVAL, "image strides smaller than the window they hold");
     GR_HIP(hipSetDevice(pl->prm.device));
@@ -455,7 +456,7 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
         const int np = std::min(pl->max_pairs, n_pairs - p0);
         {
             KernelTimer kt(pl, s, 0);
-            GR_HIP(gr_launch_rowfft(g, t, pl->prm.n_ang, sino, sino_slice_stride, sino_row_stride, n_slices, p0, np,
+            GR_HIP(gr_launch_rowfft(g, t, pl->prm.n_ang, sino, sino_slice_stride, sino_row_stride, pre, n_slices, p0, np,
                                     pl->d_spec1, pl->d_spec2, s));
             kt.done();
         }
@@ -479,6 +480,45 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
         if (pl->timing) pl->t.pairs += np;
     }
     return GR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride, float *out,
+                    long out_slice_stride, long out_row_stride, int n_slices, void *stream) {
+    if (!pl || !sino || !out) return fail(GR_EINVAL, "null argument");
+    if (n_slices < 0) return fail(GR_EINVAL, "negative slice count");
+    if (n_slices == 0) return GR_OK;
+    if (sino_row_stride < pl->g.n_det_in || sino_slice_stride < sino_row_stride * (long)pl->prm.n_ang)
+        return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
+    return recon_chunks(pl, sino, sino_slice_stride, sino_row_stride, nullptr, out, out_slice_stride, out_row_stride,
+                        n_slices, stream);
+}
+
+int gr_recon_device_raw(gr_plan *pl, const gr_preproc *pre, float *out, long out_slice_stride, long out_row_stride,
+                        int n_slices, void *stream) {
+    if (!pl || !pre || !out || !pre->raw || !pre->dark || !pre->white1) return fail(GR_EINVAL, "null argument");
+    if (pre->mode != GR_PRE_FLATDARK && pre->mode != GR_PRE_FLATDARK_LOG)
+        return fail(GR_EINVAL, "unknown preprocessing mode");
+    if (n_slices < 0) return fail(GR_EINVAL, "negative slice count");
+    if (n_slices == 0) return GR_OK;
+    if (!(fabsf(pre->xoffset) < 1e6f)) return fail(GR_EINVAL, "xoffset out of range");
+    if (pre->raw_row_stride < pl->g.n_det_in || pre->raw_slice_stride < pre->raw_row_stride * (long)pl->prm.n_ang)
+        return fail(GR_EINVAL, "count strides smaller than the rows they hold");
+    if (pre->field_stride < pl->g.n_det_in) return fail(GR_EINVAL, "field stride smaller than a row");
+    GrPre p;
+    p.mode = pre->mode;
+    p.raw = pre->raw;
+    p.raw_slice_stride = pre->raw_slice_stride;
+    p.raw_row_stride = pre->raw_row_stride;
+    p.dark = pre->dark;
+    p.white1 = pre->white1;
+    p.white2 = pre->white2 ? pre->white2 : pre->white1;
+    p.field_stride = pre->field_stride;
+    p.xoffset = pre->xoffset;
+    return recon_chunks(pl, nullptr, 0, 0, &p, out, out_slice_stride, out_row_stride, n_slices, stream);
 }
 
 int gr_recon_host(gr_plan *pl, const float *const *G1, const float *const *G2, float *const *S1,

@@ -52,9 +52,20 @@ struct GrDevTables {
 
 // K1: row FFT (+1), filter/phase, Hermitian split.  spec1/spec2 are
 // [pairs][n_ang][half] complex; pair p reads slices 2*(pair0+p) and +1.
+// Detector-count input of K1: flat/dark normalisation and sub-pixel shift
+// applied while loading (mode GR_PRE_FLATDARK or GR_PRE_FLATDARK_LOG).
+struct GrPre {
+    int mode;
+    const unsigned short *raw;
+    long raw_slice_stride, raw_row_stride;
+    const float *dark, *white1, *white2;
+    long field_stride;
+    float xoffset;
+};
+// sino is read when pre is null, pre->raw otherwise.
 hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, const float *sino,
-                            long slice_stride, long row_stride, int n_slices, int pair0, int pairs,
-                            float2 *spec1, float2 *spec2, hipStream_t s);
+                            long slice_stride, long row_stride, const GrPre *pre, int n_slices, int pair0,
+                            int pairs, float2 *spec1, float2 *spec2, hipStream_t s);
 
 // K2: tile-local gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
 // The plan lists the kernel's tiles (gr_grid_tile_size() cells per side).

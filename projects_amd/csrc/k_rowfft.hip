@@ -9,7 +9,13 @@
 //
 // The two sinogram rows stream from HBM with consecutive lanes on consecutive
 // detector cells; the half-padding caller convention (tomo_recon.cpp:3784-3807)
-// is generated while loading, so padded rows never exist in memory.  The row
+// is generated while loading, so padded rows never exist in memory.  With
+// detector counts as input (PRE != 0) the steps the callers run on the host
+// before gridrec happen here too, per value: flat/dark normalisation
+// (PRE 1: 1 - (p - d)/(w1/2 + w2/2 - d), tomo_recon.cpp:1877; PRE 2: the log
+// form log((w1/2 + w2/2 - d)/(p - d)), 0 where undefined,
+// find_shift.cpp:1685-1697, in float) and the sub-pixel shift by xoffset with
+// linear interpolation and edge extension (tomo_recon.cpp:3708-3780).  The row
 // is transformed in LDS (gr_fft.hpp) and the filtered split spectra are written
 // as two rows of pdim/2 complex values (entry 0 unused, written as zero).
 #include "gr_fft.hpp"
@@ -22,6 +28,7 @@ namespace {
 struct RowFftArgs {
     const float *sino;
     long slice_stride, row_stride;
+    GrPre pre;
     int n_slices, pair0, n_ang;
     int n_in, n_eff, pad_off, pad;
     int pdim, half, flag;
@@ -31,15 +38,41 @@ struct RowFftArgs {
     float2 *spec1, *spec2;
 };
 
-template <int MAXB>
+// Normalised value of detector cell q of one row of counts.
+template <int PRE>
+__device__ __forceinline__ float normalised(const unsigned short *raw, const float *dark, const float *w1,
+                                            const float *w2, int q) {
+    const float pj = (float)raw[q], d = dark[q], a = w1[q], b = w2[q];
+    if (PRE == 1) return 1.0f - 1.0f * (pj - d) / (0.5f * a + 0.5f * b - d);
+    const float dv = pj - d;
+    if (dv <= 0.0f) return 0.0f;
+    const float r = (0.5f * (a + b) - d) / dv;
+    return r > 0.0f ? logf(r) : 0.0f;
+}
+
+// Row value at cell k after the shift: linear interpolation at k - xoffset,
+// cells beyond either end taking the edge value (tomo_recon.cpp:3713-3778).
+template <int PRE>
+__device__ __forceinline__ float shifted(const unsigned short *raw, const float *dark, const float *w1,
+                                         const float *w2, int n, float xoffset, int k) {
+    const float kk = (float)k - xoffset;
+    const int nkk = (int)floorf(kk);
+    const int q0 = nkk < 0 ? 0 : (nkk > n - 1 ? n - 1 : nkk);
+    const int q1 = nkk + 1 < 0 ? 0 : (nkk + 1 > n - 1 ? n - 1 : nkk + 1);
+    float v = 0.0f;
+    v += normalised<PRE>(raw, dark, w1, w2, q0) * ((float)(nkk + 1) - kk);
+    v += normalised<PRE>(raw, dark, w1, w2, q1) * (kk - (float)nkk);
+    return v;
+}
+
+template <int MAXB, int PRE>
 __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     extern __shared__ float2 row[];
     const int n = blockIdx.x;
     const int p = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
-    const float *r1 = A.sino + s1 * A.slice_stride + (long)n * A.row_stride;
-    const float *r2 = s2 < A.n_slices ? A.sino + s2 * A.slice_stride + (long)n * A.row_stride : nullptr;
+    const bool has2 = s2 < A.n_slices;
 
     for (int j = tid; j < A.pdim; j += nthr) {
         float re = 0.0f, im = 0.0f;
@@ -49,8 +82,18 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
                 q = j - A.pad_off;
                 q = q < 0 ? 0 : (q > A.n_in - 1 ? A.n_in - 1 : q);
             }
-            re = r1[q];
-            if (r2) im = r2[q];
+            if (PRE == 0) {
+                re = A.sino[s1 * A.slice_stride + (long)n * A.row_stride + q];
+                if (has2) im = A.sino[s2 * A.slice_stride + (long)n * A.row_stride + q];
+            } else {
+                const GrPre &P = A.pre;
+                const long f1 = s1 * P.field_stride, f2 = s2 * P.field_stride;
+                re = shifted<PRE>(P.raw + s1 * P.raw_slice_stride + (long)n * P.raw_row_stride, P.dark + f1,
+                                  P.white1 + f1, P.white2 + f1, A.n_in, P.xoffset, q);
+                if (has2)
+                    im = shifted<PRE>(P.raw + s2 * P.raw_slice_stride + (long)n * P.raw_row_stride, P.dark + f2,
+                                      P.white1 + f2, P.white2 + f2, A.n_in, P.xoffset, q);
+            }
         }
         row[j] = make_float2(re, im);
     }
@@ -82,10 +125,12 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
 }  // namespace
 
 hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, const float *sino,
-                            long slice_stride, long row_stride, int n_slices, int pair0, int pairs,
-                            float2 *spec1, float2 *spec2, hipStream_t s) {
+                            long slice_stride, long row_stride, const GrPre *pre, int n_slices, int pair0,
+                            int pairs, float2 *spec1, float2 *spec2, hipStream_t s) {
     RowFftArgs A;
     A.sino = sino;
+    A.pre = pre ? *pre : GrPre{0, nullptr, 0, 0, nullptr, nullptr, nullptr, 0, 0.0f};
+    const int mode = pre ? pre->mode : 0;
     A.slice_stride = slice_stride;
     A.row_stride = row_stride;
     A.n_slices = n_slices;
@@ -107,11 +152,17 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     const long per = (g.pdim / 4 + nthr - 1) / nthr;
     const dim3 grid(n_ang, pairs), block(nthr);
     const size_t lds = sizeof(float2) * g.pdim;
+#define GR_LAUNCH_P(MB, PR)                                             \
+    do {                                                                \
+        hipError_t e_ = gr_allow_lds(k_rowfft<MB, PR>, lds);            \
+        if (e_ != hipSuccess) return e_;                                \
+        hipLaunchKernelGGL((k_rowfft<MB, PR>), grid, block, lds, s, A); \
+    } while (0)
 #define GR_LAUNCH(MB)                                                   \
     do {                                                                \
-        hipError_t e_ = gr_allow_lds(k_rowfft<MB>, lds);                \
-        if (e_ != hipSuccess) return e_;                                \
-        hipLaunchKernelGGL(k_rowfft<MB>, grid, block, lds, s, A);       \
+        if (mode == GR_PRE_FLATDARK) GR_LAUNCH_P(MB, 1);                \
+        else if (mode == GR_PRE_FLATDARK_LOG) GR_LAUNCH_P(MB, 2);       \
+        else GR_LAUNCH_P(MB, 0);                                        \
     } while (0)
     switch (per) {
     case 0:
@@ -123,5 +174,6 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     default: return hipErrorInvalidValue;
     }
 #undef GR_LAUNCH
+#undef GR_LAUNCH_P
     return hipGetLastError();
 }

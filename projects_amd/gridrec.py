@@ -56,6 +56,17 @@ class GrTiming(C.Structure):
     ]
 
 
+class GrPreproc(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int), ("raw", C.c_void_p), ("raw_slice_stride", C.c_long), ("raw_row_stride", C.c_long),
+        ("dark", C.c_void_p), ("white1", C.c_void_p), ("white2", C.c_void_p), ("field_stride", C.c_long),
+        ("xoffset", C.c_float),
+    ]
+
+
+PRE = {"flatdark": 1, "flatdark_log": 2}
+
+
 class GrDataset(C.Structure):
     _fields_ = [
         ("params", GrParams), ("sino", C.c_void_p), ("sino_slice_stride", C.c_long), ("sino_row_stride", C.c_long),
@@ -80,6 +91,8 @@ _SIGS = {
     "gr_recon_host": (C.c_int, [C.c_void_p] + [C.POINTER(C.POINTER(C.c_float))] * 4),
     "gr_recon_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
                                   C.c_int, C.c_void_p]),
+    "gr_recon_device_raw": (C.c_int, [C.c_void_p, C.POINTER(GrPreproc), C.c_void_p, C.c_long, C.c_long, C.c_int,
+                                      C.c_void_p]),
     "gr_plan_destroy": (C.c_int, [C.c_void_p]),
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
@@ -240,6 +253,26 @@ class Plan:
                                    C.c_void_p(out.data_ptr()), out.stride(0), out.stride(1), ns,
                                    C.c_void_p(_stream_handle(stream)))
         _check(rc, "gr_recon_device")
+        return out
+
+    def recon_raw(self, raw, dark, white1, white2=None, mode: str = "flatdark", xoffset: float = 0.0, out=None,
+                  stream=None):
+        """gridrec from detector counts (gr_recon_device_raw): raw is a cuda uint16
+        tensor [n_slices, n_ang, >=n_det]; dark/white1/white2 cuda float32
+        [n_slices, n_det] (one row per slice; white2 optional)."""
+        import torch
+
+        assert raw.is_cuda and raw.dtype == torch.uint16 and raw.dim() == 3 and raw.stride(2) == 1
+        ns, on = raw.shape[0], self.out_n
+        for f in (dark, white1) + ((white2,) if white2 is not None else ()):
+            assert f.is_cuda and f.dtype == torch.float32 and f.shape == (ns, self.n_det) and f.is_contiguous()
+        if out is None:
+            out = torch.empty((ns, on, on), dtype=torch.float32, device=raw.device)
+        pre = GrPreproc(PRE[mode], raw.data_ptr(), raw.stride(0), raw.stride(1), dark.data_ptr(), white1.data_ptr(),
+                        white2.data_ptr() if white2 is not None else None, self.n_det, xoffset)
+        rc = lib().gr_recon_device_raw(self._h, C.byref(pre), C.c_void_p(out.data_ptr()), out.stride(0),
+                                       out.stride(1), ns, C.c_void_p(_stream_handle(stream)))
+        _check(rc, "gr_recon_device_raw")
         return out
 
     def recon_host(self, g1: np.ndarray, g2: np.ndarray):

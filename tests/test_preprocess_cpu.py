@@ -1,0 +1,70 @@
+"""Pins of the oracle's restatement of the callers' preprocessing
+(oracle.preprocess: flat/dark normalisation, tomo_recon.cpp:1877 and
+find_shift.cpp:1685-1697; sub-pixel shift, tomo_recon.cpp:3708-3780), by
+closed forms: flat and dark fields map to 0 and 1, the log form inverts the
+Beer-Lambert counts, a zero shift is the identity, an integer shift moves
+cells with edge extension, and a linear row is shifted exactly.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def fields(n, rng):
+    dark = rng.uniform(90, 110, n).astype(np.float32)
+    w1 = rng.uniform(30000, 40000, n).astype(np.float32)
+    w2 = (w1 + rng.uniform(-500, 500, n)).astype(np.float32)
+    return dark, w1, w2
+
+
+def test_flat_and_dark_map_to_zero_and_one():
+    rng = np.random.default_rng(0)
+    n = 64
+    dark, w1, _ = fields(n, rng)
+    w = np.round(w1).astype(np.float32)
+    raw = np.stack([w.astype(np.uint16), np.round(dark).astype(np.uint16)])
+    d = np.round(dark).astype(np.float32)
+    out = O.preprocess(1, raw, d, w, w)
+    assert np.array_equal(out[0], np.zeros(n, np.float32)) and np.array_equal(out[1], np.ones(n, np.float32))
+
+
+def test_log_form_inverts_beer_lambert():
+    rng = np.random.default_rng(1)
+    n = 200
+    dark, w1, w2 = fields(n, rng)
+    mu = rng.uniform(0.01, 3.0, (5, n))
+    raw = np.round(dark + (0.5 * (w1 + w2) - dark) * np.exp(-mu)).astype(np.uint16)
+    v = O.preprocess(2, raw, dark, w1, w2).astype(np.float64)
+    back = dark + (0.5 * (w1.astype(np.float64) + w2) - dark) * np.exp(-v)
+    assert np.max(np.abs(back - raw) / raw) < 1e-6
+    low = raw.copy()
+    low[0, :5] = np.floor(dark[:5]).astype(np.uint16) - 1   # counts below dark
+    assert np.array_equal(O.preprocess(2, low, dark, w1, w2)[0, :5], np.zeros(5, np.float32))
+
+
+def test_zero_and_integer_shifts():
+    rng = np.random.default_rng(2)
+    n = 50
+    dark, w1, w2 = fields(n, rng)
+    raw = rng.integers(1000, 30000, (3, n)).astype(np.uint16)
+    base = O.preprocess(1, raw, dark, w1, w2)
+    for s in (-3, 2, 7):
+        out = O.preprocess(1, raw, dark, w1, w2, xoffset=float(s))
+        idx = np.clip(np.arange(n) - s, 0, n - 1)
+        assert np.array_equal(out, base[:, idx]), s
+
+
+@pytest.mark.parametrize("xoff", [0.25, -1.6, 3.5])
+def test_linear_row_shifts_exactly(xoff):
+    n = 40
+    dark = np.zeros(n, np.float32)
+    w = np.full(n, 40000.0, np.float32)
+    k = np.arange(n)
+    raw = (1000 + 500 * k)[None, :].astype(np.uint16)
+    v = O.preprocess(1, raw, dark, w, w, xoffset=xoff)[0].astype(np.float64)
+    lin = lambda x: 1.0 - (1000 + 500 * x) / 40000.0
+    interior = (k - xoff >= 0) & (k - xoff <= n - 1)
+    assert np.max(np.abs(v[interior] - lin(k[interior] - xoff))) < 1e-6
+    edge = k - xoff < 0
+    assert np.allclose(v[edge], lin(0.0), atol=1e-6)   # edge extension
