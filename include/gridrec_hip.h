@@ -19,6 +19,7 @@
  *   gr_recon_device_raw <- the same from detector counts, with the flat/dark
  *                       normalisation and sub-pixel shift of tomo_recon.cpp:1877,
  *                       3708-3780 (or find_shift.cpp:1685-1697) fused in
+ *   gr_center_entropy <- the entropy sweep of find_shift.cpp:1727-1905
  *   gr_plan_destroy  <- recon_free        grid.cpp:418-434
  *
  * All functions return 0 on success and a negative GR_E* code on failure;
@@ -182,6 +183,19 @@ typedef struct gr_preproc {
 } gr_preproc;
 int gr_recon_device_raw(gr_plan *plan, const gr_preproc *pre, float *out, long out_slice_stride,
                         long out_row_stride, int n_slices, void *stream);
+
+/* Centre sweep with the entropy metric (find_shift.cpp:1727-1905,
+ * 2078-2111): for each of n_shifts shifts, the two slices of sino (float
+ * rows, sino_slice_stride apart) are shifted by that many cells (linear
+ * interpolation, edge extension, as gr_preproc.xoffset), reconstructed with
+ * the plan (typically GR_PAD_HALF_EDGE and the n_det window, centre n_det/2),
+ * and the entropy in bits of the histogram of both images is written to
+ * entropy[i] (host): n_bins bins of width (hist_max - hist_min)/n_bins from
+ * hist_min, values in bin n_bins (the upper edge) counted, values outside
+ * ignored.  The candidates are reconstructed as one batch.  Synchronous. */
+int gr_center_entropy(gr_plan *plan, const float *sino, long sino_slice_stride, long sino_row_stride,
+                      const float *shifts, int n_shifts, float hist_min, float hist_max, int n_bins,
+                      float *entropy, void *stream);
 
 /* recon_free. */
 int gr_plan_destroy(gr_plan *plan);

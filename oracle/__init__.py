@@ -112,6 +112,8 @@ def lib():
         _lib.or_pad_half.argtypes = [C.c_int, C.c_int, fp, fp]
         _lib.or_pad_half.restype = None
         _lib.or_preprocess.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ushort), fp, fp, fp, C.c_float, fp]
+        _lib.or_entropy.argtypes = [fp, C.c_long, C.c_float, C.c_float, C.c_int]
+        _lib.or_entropy.restype = C.c_float
         _lib.or_fft_f32.argtypes = [fp, C.c_long, C.c_int]
         _lib.or_fft_f64.argtypes = [dp, C.c_long, C.c_int]
         _lib.or_recon_batch_f32.argtypes = [P, C.c_int, fp, C.c_int, C.c_long, C.c_long, fp, C.c_int]
@@ -235,6 +237,12 @@ def preprocess(mode: int, raw: np.ndarray, dark, white1, white2=None, xoffset: f
     if rc:
         raise RuntimeError("or_preprocess failed")
     return out
+
+
+def entropy(data: np.ndarray, vmin: float, vmax: float, n_bins: int) -> float:
+    """Histogram entropy in bits (find_shift.cpp:2078-2111)."""
+    d = np.ascontiguousarray(data, dtype=np.float32).ravel()
+    return float(lib().or_entropy(_fptr(d), d.size, vmin, vmax, n_bins))
 
 
 def pad_half(rows: np.ndarray) -> np.ndarray:

@@ -801,6 +801,25 @@ int or_preprocess(int mode, int n_ang, int n_det, const unsigned short *raw, con
     return 0;
 }
 
+float or_entropy(const float *data, long n, float vmin, float vmax, int n_bins) {
+    const float inv = (vmax - vmin) / (float)n_bins;
+    long *hist = (long *)calloc((size_t)n_bins + 1, sizeof(long));
+    if (!hist) return -1.0f;
+    for (long i = 0; i < n; i++) {
+        const float x = (data[i] - vmin) / inv;
+        if (!(x > -1.0f && x < (float)(n_bins + 1))) continue;  /* (int)x outside 0..n_bins */
+        hist[(int)x]++;
+    }
+    float ent = 0.0f;
+    for (int b = 0; b <= n_bins; b++) {
+        if (hist[b] <= 0) continue;
+        const float pi = 1.0f * (float)hist[b] / (float)n;
+        ent = (float)((double)ent + (double)(-pi * logf(pi)) / log(2.0));  /* float += double */
+    }
+    free(hist);
+    return ent;
+}
+
 void or_pad_half(int n_ang, int n_det, const float *src, float *dst) {
     int off = n_det / 2, w = 2 * n_det;                   /* tomo_recon.cpp:3692,3698 */
     for (int a = 0; a < n_ang; a++) {

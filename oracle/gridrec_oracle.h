@@ -101,6 +101,12 @@ void or_pad_half(int n_ang, int n_det, const float *src, float *dst);
 int or_preprocess(int mode, int n_ang, int n_det, const unsigned short *raw, const float *dark,
                   const float *white1, const float *white2, float xoffset, float *out);
 
+/* Entropy in bits of the histogram of n values (find_shift.cpp:2078-2111):
+ * bin (int)((v - min) / ((max - min) / n_bins)), counted when 0 <= bin <= n_bins;
+ * p in float, -p log p with the float log, divided by log 2 in double, summed
+ * into a float. */
+float or_entropy(const float *data, long n, float vmin, float vmax, int n_bins);
+
 /* Batch driver used for the CPU baseline: n_pairs slice pairs of the raw
  * (unpadded) sinograms sino[2*pair + {0,1}][n_ang][n_det]; when pad_half is set
  * each pair is padded with or_pad_half and reconstructed with center + n_det/2;

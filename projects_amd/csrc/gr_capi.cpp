@@ -497,6 +497,59 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
                         n_slices, stream);
 }
 
+int gr_center_entropy(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride,
+                      const float *shifts, int n_shifts, float hist_min, float hist_max, int n_bins, float *entropy,
+                      void *stream) {
+    if (!pl || !sino || !shifts || !entropy) return fail(GR_EINVAL, "null argument");
+    if (n_shifts < 1 || n_bins < 1 || n_bins > 16383 || !(hist_max > hist_min))
+        return fail(GR_EINVAL, "need n_shifts >= 1, 1 <= n_bins <= 16383 and hist_max > hist_min");
+    for (int i = 0; i < n_shifts; i++)
+        if (!(fabsf(shifts[i]) < 1e6f)) return fail(GR_EINVAL, "shift out of range");
+    const GrGeom &g = pl->g;
+    if (sino_row_stride < g.n_det_in || sino_slice_stride < sino_row_stride * (long)pl->prm.n_ang)
+        return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
+    GR_HIP(hipSetDevice(pl->prm.device));
+    hipStream_t s = (hipStream_t)stream;
+    const long on = g.out_n, nb1 = n_bins + 1;
+    float *d_shift = nullptr, *d_img = nullptr;
+    unsigned *d_hist = nullptr;
+    hipError_t e = hipMallocAsync((void **)&d_shift, sizeof(float) * n_shifts, s);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&d_img, sizeof(float) * 2 * on * on * n_shifts, s);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&d_hist, sizeof(unsigned) * nb1 * n_shifts, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_shift, shifts, sizeof(float) * n_shifts, hipMemcpyHostToDevice, s);
+    int rc = GR_OK;
+    if (e == hipSuccess) {
+        GrPre p{3, nullptr, 0, 0, nullptr, nullptr, nullptr, 0, 0.0f, d_shift};
+        rc = recon_chunks(pl, sino, sino_slice_stride, sino_row_stride, &p, d_img, on * on, on, 2 * n_shifts, s);
+    }
+    std::vector<unsigned> h((size_t)nb1 * n_shifts);
+    if (e == hipSuccess && rc == GR_OK)
+        e = gr_launch_hist(d_img, 2 * on * on, on * on, on, (int)on, n_shifts, hist_min, hist_max, n_bins, d_hist, s);
+    if (e == hipSuccess && rc == GR_OK)
+        e = hipMemcpyAsync(h.data(), d_hist, sizeof(unsigned) * h.size(), hipMemcpyDeviceToHost, s);
+    if (d_shift) (void)hipFreeAsync(d_shift, s);
+    if (d_img) (void)hipFreeAsync(d_img, s);
+    if (d_hist) (void)hipFreeAsync(d_hist, s);
+    const hipError_t w = hipStreamSynchronize(s);
+    if (rc) return rc;
+    if (e != hipSuccess) return hip_fail(e, "centre sweep");
+    if (w != hipSuccess) return hip_fail(w, "centre sweep sync");
+    // entropy in bits with the reference's arithmetic (find_shift.cpp:2101-2108):
+    // p in float, -p log p in float, divided by log 2 in double, summed into a float
+    const float npix = (float)(2 * on * on);
+    for (int c = 0; c < n_shifts; c++) {
+        float ent = 0.0f;
+        for (long b = 0; b < nb1; b++) {
+            const unsigned k = h[(size_t)c * nb1 + b];
+            if (!k) continue;
+            const float pi = 1.0f * (float)k / npix;
+            ent = (float)((double)ent + (double)(-pi * logf(pi)) / log(2.0));
+        }
+        entropy[c] = ent;
+    }
+    return GR_OK;
+}
+
 int gr_recon_device_raw(gr_plan *pl, const gr_preproc *pre, float *out, long out_slice_stride, long out_row_stride,
                         int n_slices, void *stream) {
     if (!pl || !pre || !out || !pre->raw || !pre->dark || !pre->white1) return fail(GR_EINVAL, "null argument");
@@ -518,6 +571,7 @@ int gr_recon_device_raw(gr_plan *pl, const gr_preproc *pre, float *out, long out
     p.white2 = pre->white2 ? pre->white2 : pre->white1;
     p.field_stride = pre->field_stride;
     p.xoffset = pre->xoffset;
+    p.pair_xoffset = nullptr;
     return recon_chunks(pl, nullptr, 0, 0, &p, out, out_slice_stride, out_row_stride, n_slices, stream);
 }
 

@@ -61,8 +61,11 @@ struct GrPre {
     const float *dark, *white1, *white2;
     long field_stride;
     float xoffset;
+    // mode 3 (centre sweeps): float rows of `sino`, shifted by
+    // pair_xoffset[pair] (device) instead of normalised counts
+    const float *pair_xoffset;
 };
-// sino is read when pre is null, pre->raw otherwise.
+// sino is read when pre is null or in mode 3, pre->raw otherwise.
 hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, const float *sino,
                             long slice_stride, long row_stride, const GrPre *pre, int n_slices, int pair0,
                             int pairs, float2 *spec1, float2 *spec2, hipStream_t s);
@@ -98,3 +101,9 @@ hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long 
                              unsigned long long seed, int n_ang, int n_det, const float *centers_dev,
                              int n_ell, float photons, int density, float gauss_rel, int n_tex, float tex_amp,
                              float tex_fmax, hipStream_t s);
+
+// Histograms for the entropy metric: n_cand candidates, each two on x on
+// images (slice_stride / row_stride floats apart, candidates cand_stride
+// apart); hist receives n_cand x (n_bins + 1) counts.
+hipError_t gr_launch_hist(const float *img, long cand_stride, long slice_stride, long row_stride, int on,
+                          int n_cand, float vmin, float vmax, int n_bins, unsigned *hist, hipStream_t s);

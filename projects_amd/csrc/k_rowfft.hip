@@ -51,18 +51,24 @@ __device__ __forceinline__ float normalised(const unsigned short *raw, const flo
 }
 
 // Row value at cell k after the shift: linear interpolation at k - xoffset,
-// cells beyond either end taking the edge value (tomo_recon.cpp:3713-3778).
-template <int PRE>
-__device__ __forceinline__ float shifted(const unsigned short *raw, const float *dark, const float *w1,
-                                         const float *w2, int n, float xoffset, int k) {
+// cells beyond either end taking the edge value (tomo_recon.cpp:3713-3778,
+// find_shift.cpp:1737-1778); get(q) is the unshifted value of cell q.
+template <typename Get>
+__device__ __forceinline__ float shift_interp(Get get, int n, float xoffset, int k) {
     const float kk = (float)k - xoffset;
     const int nkk = (int)floorf(kk);
     const int q0 = nkk < 0 ? 0 : (nkk > n - 1 ? n - 1 : nkk);
     const int q1 = nkk + 1 < 0 ? 0 : (nkk + 1 > n - 1 ? n - 1 : nkk + 1);
     float v = 0.0f;
-    v += normalised<PRE>(raw, dark, w1, w2, q0) * ((float)(nkk + 1) - kk);
-    v += normalised<PRE>(raw, dark, w1, w2, q1) * (kk - (float)nkk);
+    v += get(q0) * ((float)(nkk + 1) - kk);
+    v += get(q1) * (kk - (float)nkk);
     return v;
+}
+
+template <int PRE>
+__device__ __forceinline__ float shifted(const unsigned short *raw, const float *dark, const float *w1,
+                                         const float *w2, int n, float xoffset, int k) {
+    return shift_interp([&](int q) { return normalised<PRE>(raw, dark, w1, w2, q); }, n, xoffset, k);
 }
 
 template <int MAXB, int PRE>
@@ -85,6 +91,13 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
             if (PRE == 0) {
                 re = A.sino[s1 * A.slice_stride + (long)n * A.row_stride + q];
                 if (has2) im = A.sino[s2 * A.slice_stride + (long)n * A.row_stride + q];
+            } else if (PRE == 3) {
+                // every pair is the same two slices under its own shift
+                const float xo = A.pre.pair_xoffset[A.pair0 + p];
+                const float *f1 = A.sino + (long)n * A.row_stride;
+                const float *f2 = f1 + A.slice_stride;
+                re = shift_interp([&](int c) { return f1[c]; }, A.n_in, xo, q);
+                im = shift_interp([&](int c) { return f2[c]; }, A.n_in, xo, q);
             } else {
                 const GrPre &P = A.pre;
                 const long f1 = s1 * P.field_stride, f2 = s2 * P.field_stride;
@@ -129,7 +142,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
                             int pairs, float2 *spec1, float2 *spec2, hipStream_t s) {
     RowFftArgs A;
     A.sino = sino;
-    A.pre = pre ? *pre : GrPre{0, nullptr, 0, 0, nullptr, nullptr, nullptr, 0, 0.0f};
+    A.pre = pre ? *pre : GrPre{0, nullptr, 0, 0, nullptr, nullptr, nullptr, 0, 0.0f, nullptr};
     const int mode = pre ? pre->mode : 0;
     A.slice_stride = slice_stride;
     A.row_stride = row_stride;
@@ -162,6 +175,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     do {                                                                \
         if (mode == GR_PRE_FLATDARK) GR_LAUNCH_P(MB, 1);                \
         else if (mode == GR_PRE_FLATDARK_LOG) GR_LAUNCH_P(MB, 2);       \
+        else if (mode == 3) GR_LAUNCH_P(MB, 3);                         \
         else GR_LAUNCH_P(MB, 0);                                        \
     } while (0)
     switch (per) {

@@ -93,6 +93,8 @@ _SIGS = {
                                   C.c_int, C.c_void_p]),
     "gr_recon_device_raw": (C.c_int, [C.c_void_p, C.POINTER(GrPreproc), C.c_void_p, C.c_long, C.c_long, C.c_int,
                                       C.c_void_p]),
+    "gr_center_entropy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.POINTER(C.c_float), C.c_int,
+                                    C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float), C.c_void_p]),
     "gr_plan_destroy": (C.c_int, [C.c_void_p]),
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
@@ -274,6 +276,21 @@ class Plan:
                                        out.stride(1), ns, C.c_void_p(_stream_handle(stream)))
         _check(rc, "gr_recon_device_raw")
         return out
+
+    def center_entropy(self, sino, shifts, hist_min: float, hist_max: float, n_bins: int, stream=None) -> np.ndarray:
+        """Entropy (bits) of the reconstruction of the two slices of sino (cuda
+        float32 [2, n_ang, >=n_det]) under each row shift (gr_center_entropy)."""
+        import torch
+
+        assert sino.is_cuda and sino.dtype == torch.float32 and sino.dim() == 3 and sino.shape[0] == 2
+        sh = np.ascontiguousarray(np.asarray(shifts, np.float32))
+        ent = np.empty(sh.shape[0], np.float32)
+        fp = C.POINTER(C.c_float)
+        rc = lib().gr_center_entropy(self._h, C.c_void_p(sino.data_ptr()), sino.stride(0), sino.stride(1),
+                                     sh.ctypes.data_as(fp), sh.shape[0], hist_min, hist_max, n_bins,
+                                     ent.ctypes.data_as(fp), C.c_void_p(_stream_handle(stream)))
+        _check(rc, "gr_center_entropy")
+        return ent
 
     def recon_host(self, g1: np.ndarray, g2: np.ndarray):
         """do_recon on host rows (gr_recon_host): two out_n x out_n float32 images."""

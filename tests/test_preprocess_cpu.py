@@ -68,3 +68,16 @@ def test_linear_row_shifts_exactly(xoff):
     assert np.max(np.abs(v[interior] - lin(k[interior] - xoff))) < 1e-6
     edge = k - xoff < 0
     assert np.allclose(v[edge], lin(0.0), atol=1e-6)   # edge extension
+
+
+def test_entropy_closed_forms():
+    assert O.entropy(np.zeros(1000, np.float32), -1.0, 1.0, 10) == 0.0           # one bin
+    half = np.r_[np.full(500, 0.05), np.full(500, 0.95)].astype(np.float32)
+    assert abs(O.entropy(half, 0.0, 1.0, 10) - 1.0) < 1e-6                         # two equal bins
+    u = (np.arange(4096, dtype=np.float32) + 0.5) / 4096
+    assert abs(O.entropy(u, 0.0, 1.0, 64) - 6.0) < 1e-5                            # uniform over 64 bins
+    # outside the range not counted but still in the total; the upper edge
+    # counts in bin n_bins; values within one bin below the lower edge truncate
+    # into bin 0, as (int) does: three singleton bins out of five values
+    x = np.array([0.5, 2.0, -5.0, 1.0, -0.05], np.float32)
+    assert abs(O.entropy(x, 0.0, 1.0, 10) - 3 * (-0.2 * np.log2(0.2))) < 1e-6
