@@ -29,8 +29,12 @@
 // pairs' spectra of that range in its own LDS slot (64 samples at a time) and
 // each lane then runs its own exact candidate range against LDS.
 //
-// Accumulation: GR_ACCUM_COMPENSATED keeps a Kahan correction per component;
-// GR_ACCUM_PLAIN adds in float like the reference.
+// Accumulation: GR_ACCUM_PLAIN adds each tap's float product in float, in
+// the reference's order.  GR_ACCUM_COMPENSATED sums the (at most a few) taps
+// one sample line gives a cell with fused multiply-adds into a per-angle
+// partial, and adds each angle's partial into the cell's running sum with an
+// error-free TwoSum (Knuth), carrying the rounding errors in a second float:
+// the cell's value is the sum plus the carried error, rounded once.
 //
 // Output: Ht[pair][b][a] = H[a][b] (transposed, so K3's rows run over a).
 #include "gr_fft.hpp"
@@ -56,15 +60,12 @@ struct GridArgs {
     float L2m, dmax, rmax;
 };
 
-template <int ACCUM>
-__device__ __forceinline__ void accumulate(float4 &acc, float xr, float xi) {
-    if (ACCUM == GR_ACCUM_COMPENSATED) {
-        gr_kahan(acc.x, acc.y, xr);
-        gr_kahan(acc.z, acc.w, xi);
-    } else {
-        acc.x += xr;
-        acc.z += xi;
-    }
+// s + c += x with s the float sum and c the carried rounding errors.
+__device__ __forceinline__ void two_sum_add(float &s, float &c, float x) {
+    const float t = s + x;
+    const float bp = t - s;
+    c += (s - (t - bp)) + (x - bp);
+    s = t;
 }
 
 // Stage samples c0 .. c0+nj-1 of angle n for the P pairs, then add the taps
@@ -74,7 +75,8 @@ __device__ __forceinline__ void accumulate(float4 &acc, float xr, float xi) {
 template <int ACCUM, int P>
 __device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int nvalid, long row_off, int c0, int nj, int j0,
                                       int j1, float C, float S, float qa, float qb, const GrGatherGeom &g,
-                                      const float *wtbl, float2 (*stage)[64], int lane, float4 (&acc)[P]) {
+                                      const float *wtbl, float2 (*stage)[64], int lane, float4 (&acc)[P],
+                                      float2 (&part)[P]) {
 #pragma unroll
     for (int p = 0; p < P; p++)
         if (lane < nj) stage[p][lane] = spec[(p < nvalid ? p : nvalid - 1) * pair_stride + row_off + c0 + lane];
@@ -85,7 +87,13 @@ __device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int 
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 const float2 v = stage[p][j - c0];
-                accumulate<ACCUM>(acc[p], w * v.x, w * v.y);
+                if (ACCUM == GR_ACCUM_COMPENSATED) {
+                    part[p].x = __builtin_fmaf(w, v.x, part[p].x);
+                    part[p].y = __builtin_fmaf(w, v.y, part[p].y);
+                } else {
+                    acc[p].x += w * v.x;
+                    acc[p].z += w * v.y;
+                }
             }
         }
     }
@@ -116,9 +124,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
     const float du = pa - M2, dv = pb - M2;
     const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
-    float4 acc[P];
+    float4 acc[P];   // (sum re, error re, sum im, error im)
+    float2 part[P];  // compensated mode: this angle's partial
 #pragma unroll
-    for (int p = 0; p < P; p++) acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int p = 0; p < P; p++) {
+        acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        part[p] = make_float2(0.0f, 0.0f);
+    }
 
     if (sqrtf(cu * cu + cv * cv) <= A.rmax) {
         int r0, r1, r2, r3;
@@ -126,8 +138,8 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
         const long pair_stride = (long)A.n_ang * A.half;
         const float2 *base1 = A.spec1 + pair0 * pair_stride;
         const float2 *base2 = A.spec2 + pair0 * pair_stride;
-        for (int part = 0; part < 2; part++) {
-            const int k0 = part ? r2 : r0, k1 = part ? r3 : r1;
+        for (int side = 0; side < 2; side++) {
+            const int k0 = side ? r2 : r0, k1 = side ? r3 : r1;
             for (int k = k0; k < k1; k++) {
                 const float4 cs = A.ang[k];
                 // does the line pass through the tile grown by the box half-width?
@@ -144,10 +156,20 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
                 const long row_off = (long)A.aord[k] * A.half;
                 for (int c0 = td0; c0 <= td1; c0 += 64)
                     chunk<ACCUM, P>(base1, pair_stride, nvalid, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0),
-                                    min(min(jd1, td1), c0 + 63), cs.x, cs.y, pa, pb, A.g, s_wtbl, stage, lane, acc);
+                                    min(min(jd1, td1), c0 + 63), cs.x, cs.y, pa, pb, A.g, s_wtbl, stage, lane, acc,
+                                    part);
                 for (int c0 = tm0; c0 <= tm1; c0 += 64)
                     chunk<ACCUM, P>(base2, pair_stride, nvalid, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0),
-                                    min(min(jm1, tm1), c0 + 63), cs.x, cs.y, ma, mb, A.g, s_wtbl, stage, lane, acc);
+                                    min(min(jm1, tm1), c0 + 63), cs.x, cs.y, ma, mb, A.g, s_wtbl, stage, lane, acc,
+                                    part);
+                if (ACCUM == GR_ACCUM_COMPENSATED) {
+#pragma unroll
+                    for (int p = 0; p < P; p++) {
+                        two_sum_add(acc[p].x, acc[p].y, part[p].x);
+                        two_sum_add(acc[p].z, acc[p].w, part[p].y);
+                        part[p] = make_float2(0.0f, 0.0f);
+                    }
+                }
             }
         }
     }
@@ -156,7 +178,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
         for (int p = 0; p < P; p++) {
             if (p >= nvalid) break;
             const float4 s = acc[p];
-            const float2 h = ACCUM == GR_ACCUM_COMPENSATED ? make_float2(s.x - s.y, s.z - s.w) : make_float2(s.x, s.z);
+            const float2 h = ACCUM == GR_ACCUM_COMPENSATED ? make_float2(s.x + s.y, s.z + s.w) : make_float2(s.x, s.z);
             A.ht[((long)(pair0 + p) * M + b) * M + a] = h;
         }
     }
