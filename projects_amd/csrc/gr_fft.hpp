@@ -59,7 +59,7 @@ __device__ __forceinline__ void gr_dft16(float2 (&v)[16]) {
 }
 
 template <int SIGN, int MAXB>
-__device__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr) {
+__device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr) {
     int Ns = 1;
     constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
     const int nb16 = N >> 4;

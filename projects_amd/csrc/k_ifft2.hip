@@ -117,14 +117,17 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
     return hipGetLastError();
 }
 
-// Columns per workgroup: as many M-point columns as fit in 128 KiB of LDS,
-// halved while the launch would hold fewer than 8 workgroups per CU (small
-// windows or few pairs leave most of the 256 CUs idle otherwise).
+// Columns per workgroup by default: one.  Wider groups read Z in longer
+// contiguous runs but hold cw x M x 8 bytes of LDS, which caps the
+// workgroups per CU; with the column groups numbered per XCD (neighbouring
+// columns share an L2) one column per workgroup measured fastest for M = 1024
+// .. 8192 on MI355X, as the queue's autotuner also finds.  Callers may ask for
+// a width (gr_plan.cols_per_block).
 static int cols_per_block(long M, long out_n, int pairs) {
-    long cw = 16384 / M;
-    cw = cw < 1 ? 1 : (cw > 16 ? 16 : cw);
-    while (cw > 1 && ((out_n + cw - 1) / cw) * pairs < 2048) cw >>= 1;
-    return (int)cw;
+    (void)M;
+    (void)out_n;
+    (void)pairs;
+    return 1;
 }
 
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
