@@ -30,11 +30,12 @@
 // each lane then runs its own exact candidate range against LDS.
 //
 // Accumulation: GR_ACCUM_PLAIN adds each tap's float product in float, in
-// the reference's order.  GR_ACCUM_COMPENSATED sums the (at most a few) taps
-// one sample line gives a cell with fused multiply-adds into a per-angle
-// partial, and adds each angle's partial into the cell's running sum with an
-// error-free TwoSum (Knuth), carrying the rounding errors in a second float:
-// the cell's value is the sum plus the carried error, rounded once.
+// the reference's order.  GR_ACCUM_COMPENSATED sums the taps of up to
+// kMergeAngles consecutive sample lines (a few taps each) with fused
+// multiply-adds into a partial, and adds each partial into the cell's running
+// sum with an error-free TwoSum (Knuth), carrying the rounding errors in a
+// second float: the cell's value is the sum plus the carried error, rounded
+// once.
 //
 // Output: Ht[pair][b][a] = H[a][b] (transposed, so K3's rows run over a).
 #include "gr_fft.hpp"
@@ -45,7 +46,8 @@
 
 namespace {
 
-constexpr int kTile = 8;  // cells per tile side; one tile per wavefront
+constexpr int kTile = 8;        // cells per tile side; one tile per wavefront
+constexpr int kMergeAngles = 4;  // sample lines per compensated partial
 
 struct GridArgs {
     const float2 *spec1, *spec2;
@@ -66,6 +68,17 @@ __device__ __forceinline__ void two_sum_add(float &s, float &c, float x) {
     const float bp = t - s;
     c += (s - (t - bp)) + (x - bp);
     s = t;
+}
+
+// Close the open partial of every pair into its compensated sum.
+template <int P>
+__device__ __forceinline__ void merge(float4 (&acc)[P], float2 (&part)[P]) {
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        two_sum_add(acc[p].x, acc[p].y, part[p].x);
+        two_sum_add(acc[p].z, acc[p].w, part[p].y);
+        part[p] = make_float2(0.0f, 0.0f);
+    }
 }
 
 // Stage samples c0 .. c0+nj-1 of angle n for the P pairs, then add the taps
@@ -133,6 +146,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     }
 
     if (sqrtf(cu * cu + cv * cv) <= A.rmax) {
+        int lines = 0;  // sample lines in the open partial
         int r0, r1, r2, r3;
         gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
         const long pair_stride = (long)A.n_ang * A.half;
@@ -162,16 +176,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
                     chunk<ACCUM, P>(base2, pair_stride, nvalid, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0),
                                     min(min(jm1, tm1), c0 + 63), cs.x, cs.y, ma, mb, A.g, s_wtbl, stage, lane, acc,
                                     part);
-                if (ACCUM == GR_ACCUM_COMPENSATED) {
-#pragma unroll
-                    for (int p = 0; p < P; p++) {
-                        two_sum_add(acc[p].x, acc[p].y, part[p].x);
-                        two_sum_add(acc[p].z, acc[p].w, part[p].y);
-                        part[p] = make_float2(0.0f, 0.0f);
-                    }
+                if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
+                    merge<P>(acc, part);
+                    lines = 0;
                 }
             }
         }
+        if (ACCUM == GR_ACCUM_COMPENSATED && lines) merge<P>(acc, part);
     }
     if (a < M && b < M) {
 #pragma unroll
