@@ -424,24 +424,39 @@ __device__ __forceinline__ void fused_conv32(const uint16_t *in, int Ci, uint16_
             A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
     const int j = lane & 15, g = lane >> 4;
     const int npix = Ro * Co, ngroups = (npix + 15) / 16;
-    for (int grp = wave; grp < ngroups; grp += nwaves) {
-        const int p = grp * 16 + j;
-        const int pc = p < npix ? p : npix - 1;
-        const int r = pc / Co, c = pc - r * Co;
-        floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+    // two 16-pixel groups per trip: four independent accumulation chains
+    for (int grp0 = 2 * wave; grp0 < ngroups; grp0 += 2 * nwaves) {
+        int pp[2], rr[2], cc[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            pp[h] = (grp0 + h) * 16 + j;
+            const int pc = pp[h] < npix ? pp[h] : npix - 1;
+            rr[h] = pc / Co;
+            cc[h] = pc - rr[h] * Co;
+        }
+        floatx4 acc[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) acc[h][0] = acc[h][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int t = 0; t < 9; t++) {
             const int ky = t / 3, kx = t - ky * 3;
-            const bf16x8 B = *reinterpret_cast<const bf16x8 *>(in + ((r + ky) * Ci + c + kx) * PSTR + 8 * g);
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][t], B, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][t], B, acc1, 0, 0, 0);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const bf16x8 B =
+                    *reinterpret_cast<const bf16x8 *>(in + ((rr[h] + ky) * Ci + cc[h] + kx) * PSTR + 8 * g);
+                acc[h][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][t], B, acc[h][0], 0, 0, 0);
+                acc[h][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][t], B, acc[h][1], 0, 0, 0);
+            }
         }
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int p = pp[h], r = rr[h], c = cc[h];
         if (p < npix) {
             const int gy = oy + r, gx = ox + c;
             const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
 #pragma unroll
             for (int nb = 0; nb < 2; nb++) {
-                const floatx4 a = nb ? acc1 : acc0;
+                const floatx4 a = acc[h][nb];
                 const int co0 = nb * 16 + 4 * g;
                 float v[4];
 #pragma unroll
@@ -455,6 +470,7 @@ __device__ __forceinline__ void fused_conv32(const uint16_t *in, int Ci, uint16_
                 *reinterpret_cast<uint2 *>(out + p * PSTR + co0) = o;
             }
         }
+      }
     }
 }
 
