@@ -108,3 +108,26 @@ def test_null_arguments():
     assert L.gr_recon_device(None, None, 0, 0, None, 0, 0, 1, None) == gr.GR_EINVAL
     assert L.gr_plan_destroy(None) == gr.GR_OK
     assert L.gr_get_pswf(C.c_float(6.0), None) == gr.GR_EINVAL
+
+
+def test_null_and_invalid_arguments_of_the_extensions():
+    """Queue, detector-count input, centre sweep and CNN entry points reject
+    bad arguments before any device work."""
+    L = gr.lib()
+    assert L.gr_queue_create(None, 0, 0, 1, 1, None) == gr.GR_EINVAL
+    assert L.gr_queue_run(None, None) == gr.GR_EINVAL
+    assert L.gr_queue_destroy(None) == gr.GR_OK
+    assert L.gr_recon_device_raw(None, None, None, 0, 0, 1, None) == gr.GR_EINVAL
+    sh = (C.c_float * 1)(0.0)
+    ent = (C.c_float * 1)()
+    assert L.gr_center_entropy(None, None, 0, 0, sh, 1, 0.0, 1.0, 10, ent, None) == gr.GR_EINVAL
+    assert L.gr_cnn_param_count() == 28353
+    assert L.gr_cnn_packed_count() == 3 * 2 * 9 * 32 * 32
+    assert L.gr_cnn_pack(None, None, None) == gr.GR_EINVAL
+    assert L.gr_cnn_forward(None, None, None, 1, 8, 8, 1.0, *([None] * 9)) == gr.GR_EINVAL
+    assert L.gr_cnn_infer(None, None, None, 1, 8, 8, 1.0, *([None] * 4)) == gr.GR_EINVAL
+    assert L.gr_cnn_backward(None, None, None, None, 1, 8, 8, 1.0, *([None] * 9)) == gr.GR_EINVAL
+    assert L.gr_cnn_adam(None, None, None, None, 10, 1, 1e-3, 0.9, 0.999, 1e-8, None) == gr.GR_EINVAL
+    # work sizes grow with the batch and cover the tile partials
+    assert L.gr_cnn_work_bytes(2, 64, 64) > L.gr_cnn_work_bytes(1, 64, 64) > 0
+    assert L.gr_cnn_infer_work_bytes(1, 16, 32) == 8  # one tile: two floats
