@@ -225,8 +225,8 @@ int gr_phantom_sinograms(float *sino, long slice_stride, int n_slices, long slic
  * dataset; tomo_recon.cpp:3700-3863 once per slice batch) made one launch.
  * Each dataset has its own gr_params and buffers (arguments as in
  * gr_recon_device).  gr_queue_create builds one plan per dataset, autotunes
- * slice pairs per launch and the column-FFT width per size class (when
- * autotune != 0), spreads the datasets over n_streams streams (1..8) and
+ * slice pairs per launch, the column-FFT width, the largest FFT radix and
+ * the gridding workgroup size per size class (when autotune != 0), spreads the datasets over n_streams streams (1..8) and
  * captures all their launches into one HIP graph; it runs the batch while
  * tuning, so the output buffers are written.  gr_queue_run replays the graph
  * on `stream`, asynchronously; the images
@@ -257,9 +257,10 @@ int gr_queue_run(gr_queue *queue, void *stream);
 int gr_queue_run_sequential(gr_queue *queue, void *stream);
 int gr_queue_get_info(const gr_queue *queue, gr_queue_info *info);
 /* Tuned settings of dataset d: pairs per launch, column-FFT width (0 =
- * automatic), stream index.  Any pointer may be NULL. */
+ * automatic), largest FFT radix (16 or 4), gridding wavefronts per workgroup,
+ * stream index.  Any pointer may be NULL. */
 int gr_queue_dataset_settings(const gr_queue *queue, int d, int *max_pairs, int *cols_per_block,
-                              int *stream);
+                              int *fft_radix, int *grid_wpb, int *stream);
 int gr_queue_destroy(gr_queue *queue);
 
 const char *gr_last_error(void);

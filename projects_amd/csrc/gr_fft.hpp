@@ -3,7 +3,8 @@
 // Stockham autosort formulation (N. K. Govindaraju et al., "High performance
 // discrete Fourier transforms on graphics processors", SC'08): passes of
 // radix 16 (a 4 x 4 decomposition in registers) while 16 divides what is left,
-// then radix 4, then one radix-2 pass when needed; output in natural order.
+// then radix 4, then one radix-2 pass when needed (max_radix 4 skips the
+// radix-16 passes: a launch-tuning choice); output in natural order.
 // Each pass reads all of a thread's butterfly inputs into registers, crosses a
 // barrier, and writes the outputs back into the same LDS row, so one buffer of
 // N complex values suffices.  SIGN = +1 computes sum_k x_k e^{+2 pi i jk/N},
@@ -59,11 +60,12 @@ __device__ __forceinline__ void gr_dft16(float2 (&v)[16]) {
 }
 
 template <int SIGN, int MAXB>
-__device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr) {
+__device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr,
+                                           int max_radix = 16) {
     int Ns = 1;
     constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
     const int nb16 = N >> 4;
-    while (Ns * 16 <= N) {
+    while (max_radix >= 16 && Ns * 16 <= N) {
         float2 v[MAXB16][16];
         const int stride = N / (Ns * 16);
 #pragma unroll

@@ -21,6 +21,9 @@ struct GrGeom {
     int filter_id;
     gr_pswf pswf;
     long out_off, out_n;          // output window inside M0 x M0
+    // launch tuning (no effect on results): largest FFT radix (16 or 4) and
+    // wavefronts (8x8 tiles) per gridding workgroup (1, 2 or 4)
+    int fft_radix = 16, grid_wpb = 2;
 };
 
 // Host side (gr_tables.cpp): sizes and the reference's float tables.

@@ -8,16 +8,17 @@
 //
 //  * one plan per dataset (its own tables), with workspaces shared per stream
 //    instead of one per plan;
-//  * an autotuner: for every size class (pdim, M, output window) it times the
-//    candidate pairs-per-launch and column-FFT widths on the class's first
-//    dataset and keeps the fastest;
+//  * an autotuner: for every size class (pdim, M, output window) it times, on
+//    the class's largest dataset, the slice pairs per launch, the column-FFT
+//    width, the largest FFT radix and the gridding workgroup size (coordinate
+//    descent) and keeps the fastest;
 //  * datasets are spread over several streams, heaviest first onto the least
 //    loaded stream, and all their launches are captured into one graph whose
 //    branches fork from and join back into the caller's stream, so small
 //    datasets run side by side and the host issues one launch per batch.
 //
 // A queue's results are bitwise those of per-dataset gr_recon_device calls:
-// the same kernels, and neither tuning knob changes any sum order.
+// the same kernels, and no tuning knob changes any sum order.
 #include <algorithm>
 #include <map>
 #include <numeric>
@@ -169,45 +170,63 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
             if (q->ds[d].n_slices > q->ds[d0].n_slices) d0 = d;
         gr_plan *p = q->plans[d0];
         const int pairs = (q->ds[d0].n_slices + 1) / 2;
-        int best_mp = std::min(pairs, 8), best_cw = 0;
+        int best_mp = std::min(pairs, 8), best_cw = 0, best_radix = 16, best_wpb = 4;
         float best_ms = 0.0f;
+        // one timing of the class's dataset with a candidate setting
+        auto trial = [&](int mp, int cw, int radix, int wpb, float *ms) {
+            gr_plan_bind_workspace(p, tune_ws, mp);
+            p->cols_per_block = cw;
+            p->g.fft_radix = radix;
+            p->g.grid_wpb = wpb;
+            tuned++;
+            return time_dataset(q, d0, s0, q->events[0], q->events[1], ms);
+        };
         if (autotune) {
+            // coordinate descent: pairs per launch, then the column-FFT width,
+            // the largest FFT radix and the gridding workgroup size in turn,
+            // each with the others at their best so far
             best_ms = 1e30f;
             const long M = p->g.M;
             std::vector<int> cws = {0};
             for (int c = 1; c <= 16 && c * M <= 16384; c *= 2) cws.push_back(c);
-            for (int mp : kPairChoices) {
-                if (mp > pairs && mp != 1) continue;
-                for (int cw : cws) {
-                    gr_plan_bind_workspace(p, tune_ws, mp);
-                    p->cols_per_block = cw;
-                    float ms = 0.0f;
-                    rc = time_dataset(q, d0, s0, q->events[0], q->events[1], &ms);
-                    if (rc) {
-                        (void)hipFree(tune_ws);
-                        free_queue(q);
-                        return rc;
+            const int radixes[] = {16, 4}, wpbs[] = {4, 2, 1};
+            for (int stage = 0; stage < 4 && !rc; stage++) {
+                std::vector<int> cand;
+                if (stage == 0)
+                    for (int mp : kPairChoices) {
+                        if (mp <= pairs || mp == 1) cand.push_back(mp);
                     }
-                    tuned++;
+                if (stage == 1) cand = cws;
+                if (stage == 2) cand.assign(std::begin(radixes), std::end(radixes));
+                if (stage == 3) cand.assign(std::begin(wpbs), std::end(wpbs));
+                for (int v : cand) {
+                    int mp = best_mp, cw = best_cw, radix = best_radix, wpb = best_wpb;
+                    (stage == 0 ? mp : stage == 1 ? cw : stage == 2 ? radix : wpb) = v;
+                    float ms = 0.0f;
+                    rc = trial(mp, cw, radix, wpb, &ms);
+                    if (rc) break;
                     if (ms < best_ms) {
                         best_ms = ms;
                         best_mp = mp;
                         best_cw = cw;
+                        best_radix = radix;
+                        best_wpb = wpb;
                     }
                 }
             }
         } else {
-            gr_plan_bind_workspace(p, tune_ws, best_mp);
-            rc = time_dataset(q, d0, s0, q->events[0], q->events[1], &best_ms);
-            if (rc) {
-                (void)hipFree(tune_ws);
-                free_queue(q);
-                return rc;
-            }
+            rc = trial(best_mp, best_cw, best_radix, best_wpb, &best_ms);
+        }
+        if (rc) {
+            (void)hipFree(tune_ws);
+            free_queue(q);
+            return rc;
         }
         for (int d : kv.second) {
             q->plans[d]->max_pairs = best_mp;
             q->plans[d]->cols_per_block = best_cw;
+            q->plans[d]->g.fft_radix = best_radix;
+            q->plans[d]->g.grid_wpb = best_wpb;
         }
     }
     // one pass over every dataset with its tuned settings before capture: the
@@ -323,10 +342,13 @@ int gr_queue_get_info(const gr_queue *q, gr_queue_info *info) {
     return GR_OK;
 }
 
-int gr_queue_dataset_settings(const gr_queue *q, int d, int *max_pairs, int *cols_per_block, int *stream) {
+int gr_queue_dataset_settings(const gr_queue *q, int d, int *max_pairs, int *cols_per_block, int *fft_radix,
+                              int *grid_wpb, int *stream) {
     if (!q || d < 0 || d >= (int)q->ds.size()) return gr_set_error(GR_EINVAL, "invalid dataset index");
     if (max_pairs) *max_pairs = q->plans[d]->max_pairs;
     if (cols_per_block) *cols_per_block = q->plans[d]->cols_per_block;
+    if (fft_radix) *fft_radix = q->plans[d]->g.fft_radix;
+    if (grid_wpb) *grid_wpb = q->plans[d]->g.grid_wpb;
     if (stream) *stream = q->stream_of[d];
     return GR_OK;
 }

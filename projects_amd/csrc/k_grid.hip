@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * 4 + wave;
+    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
     if (ti >= A.ntiles) return;
     float2(*stage)[64] = reinterpret_cast<float2(*)[64]>(smem + A.lds_wtbl) + wave * P;
     const int pair0 = A.pair_base + blockIdx.y * P;
@@ -196,16 +196,16 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
 }
 
 template <int ACCUM, int P>
-hipError_t launch(GridArgs A, int groups, int ntiles, size_t lds_base, hipStream_t s) {
-    const size_t lds = lds_base + sizeof(float2) * 64 * 4 * P;
+hipError_t launch(GridArgs A, int groups, int ntiles, size_t lds_base, int wpb, hipStream_t s) {
+    const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
     hipError_t e = gr_allow_lds(k_grid<ACCUM, P>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3((ntiles + 3) / 4, groups), dim3(256), lds, s, A);
+    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3((ntiles + wpb - 1) / wpb, groups), dim3(64 * wpb), lds, s, A);
     return hipGetLastError();
 }
 
 template <int ACCUM>
-hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, hipStream_t s) {
+hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, int wpb, hipStream_t s) {
     // groups of 8 pairs; a remainder of 5-7 rides in one more (partial) group
     // of 8, one of 3-4 in a group of 4, 2 and 1 in their own widths: a tap's
     // geometry costs the same for any width, so one partial wide group beats
@@ -214,12 +214,12 @@ hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, hipStr
     hipError_t e = hipSuccess;
     A.pair_end = pairs;
     A.pair_base = 0;
-    if (full + (rem >= 5) > 0) e = launch<ACCUM, 8>(A, full + (rem >= 5), ntiles, lds_base, s);
+    if (full + (rem >= 5) > 0) e = launch<ACCUM, 8>(A, full + (rem >= 5), ntiles, lds_base, wpb, s);
     if (e != hipSuccess || rem == 0 || rem >= 5) return e;
     A.pair_base = full * 8;
-    if (rem >= 3) return launch<ACCUM, 4>(A, 1, ntiles, lds_base, s);
-    if (rem == 2) return launch<ACCUM, 2>(A, 1, ntiles, lds_base, s);
-    return launch<ACCUM, 1>(A, 1, ntiles, lds_base, s);
+    if (rem >= 3) return launch<ACCUM, 4>(A, 1, ntiles, lds_base, wpb, s);
+    if (rem == 2) return launch<ACCUM, 2>(A, 1, ntiles, lds_base, wpb, s);
+    return launch<ACCUM, 1>(A, 1, ntiles, lds_base, wpb, s);
 }
 
 }  // namespace
@@ -255,6 +255,7 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
     A.rmax = g.scale * (float)(g.half - 1) * 1.0001f + A.dmax + 0.5f;
     const size_t lds_base = sizeof(float) * A.lds_wtbl;
-    if (accum == GR_ACCUM_PLAIN) return launch_all<GR_ACCUM_PLAIN>(A, pairs, t.ntiles, lds_base, s);
-    return launch_all<GR_ACCUM_COMPENSATED>(A, pairs, t.ntiles, lds_base, s);
+    const int wpb = g.grid_wpb == 1 || g.grid_wpb == 2 ? g.grid_wpb : 4;
+    if (accum == GR_ACCUM_PLAIN) return launch_all<GR_ACCUM_PLAIN>(A, pairs, t.ntiles, lds_base, wpb, s);
+    return launch_all<GR_ACCUM_COMPENSATED>(A, pairs, t.ntiles, lds_base, wpb, s);
 }

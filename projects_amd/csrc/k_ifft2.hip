@@ -26,7 +26,7 @@ struct RowsArgs {
     const float2 *ht;
     float2 *z;
     const float2 *tw;
-    int M;
+    int M, radix;
     long M02, out_off, out_n;
 };
 
@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(512) k_ifft_rows(RowsArgs A) {
     const float2 *src = A.ht + ((long)p * A.M + b) * A.M;
     for (int i = tid; i < A.M; i += nthr) row[i] = src[i];
     __syncthreads();
-    gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr);
+    gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr, A.radix);
     float2 *dst = A.z + ((long)p * A.M + b) * A.out_n;
     for (long jj = tid; jj < A.out_n; jj += nthr) dst[jj] = row[corner_index(A.out_off + jj, A.M, A.M02)];
 }
@@ -49,7 +49,7 @@ struct ColsArgs {
     const float2 *tw;
     const float *winv;
     long out_slice_stride, out_row_stride;
-    int M, cw, n_slices, pair0;
+    int M, cw, n_slices, pair0, radix;
     long M02, out_off, out_n;
 };
 
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
         cols[c * M + b] = (c0 + c < A.out_n) ? zp[b * A.out_n + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + (long)c * M, M, A.tw, tid, nthr);
+    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + (long)c * M, M, A.tw, tid, nthr, A.radix);
 
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     float *o1 = A.out + s1 * A.out_slice_stride;
@@ -101,7 +101,7 @@ inline int maxb_of(long N, int nthr) { return (int)((N / 4 + nthr - 1) / nthr); 
 
 hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
                                int pairs, hipStream_t s) {
-    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.M02, g.out_off, g.out_n};
+    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n};
     const int nthr = gr_fft_threads(g.M);
     const dim3 grid((unsigned)g.M, pairs), block(nthr);
     const size_t lds = sizeof(float2) * g.M;
@@ -144,6 +144,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.cw = cols > 0 && (long)cols * g.M <= 16384 ? cols : cols_per_block(g.M, g.out_n, pairs);
     A.n_slices = n_slices;
     A.pair0 = pair0;
+    A.radix = g.fft_radix;
     A.M02 = g.M02;
     A.out_off = g.out_off;
     A.out_n = g.out_n;

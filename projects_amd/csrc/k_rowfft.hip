@@ -31,7 +31,7 @@ struct RowFftArgs {
     GrPre pre;
     int n_slices, pair0, n_ang;
     int n_in, n_eff, pad_off, pad;
-    int pdim, half, flag;
+    int pdim, half, flag, radix;
     const float2 *filphase;
     const float *roi_offset;
     const float2 *tw;
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
         row[j] = make_float2(re, im);
     }
     __syncthreads();
-    gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr);
+    gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr, A.radix);
 
     float2 *o1 = A.spec1 + ((long)p * A.n_ang + n) * A.half;
     float2 *o2 = A.spec2 + ((long)p * A.n_ang + n) * A.half;
@@ -156,6 +156,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     A.pdim = (int)g.pdim;
     A.half = (int)g.half;
     A.flag = g.flag;
+    A.radix = g.fft_radix;
     A.filphase = t.filphase;
     A.roi_offset = t.roi_offset;
     A.tw = t.tw_row;

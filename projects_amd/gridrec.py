@@ -106,8 +106,7 @@ _SIGS = {
     "gr_queue_run": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_queue_run_sequential": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_queue_get_info": (C.c_int, [C.c_void_p, C.POINTER(GrQueueInfo)]),
-    "gr_queue_dataset_settings": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                            C.POINTER(C.c_int)]),
+    "gr_queue_dataset_settings": (C.c_int, [C.c_void_p, C.c_int] + [C.POINTER(C.c_int)] * 5),
     "gr_queue_destroy": (C.c_int, [C.c_void_p]),
     # include/gridrec_cnn.h
     "gr_cnn_param_count": (C.c_long, []),
@@ -356,10 +355,11 @@ class Queue:
                "gr_queue_run_sequential")
 
     def settings(self, d: int) -> dict:
-        mp, cw, st = C.c_int(), C.c_int(), C.c_int()
-        _check(lib().gr_queue_dataset_settings(self._h, d, C.byref(mp), C.byref(cw), C.byref(st)),
-               "gr_queue_dataset_settings")
-        return dict(max_pairs=mp.value, cols_per_block=cw.value, stream=st.value)
+        mp, cw, rx, wpb, st = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().gr_queue_dataset_settings(self._h, d, C.byref(mp), C.byref(cw), C.byref(rx), C.byref(wpb),
+                                               C.byref(st)), "gr_queue_dataset_settings")
+        return dict(max_pairs=mp.value, cols_per_block=cw.value, fft_radix=rx.value, grid_wpb=wpb.value,
+                    stream=st.value)
 
     def close(self):
         if getattr(self, "_h", None):

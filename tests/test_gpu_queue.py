@@ -1,10 +1,12 @@
 """GPU tests of the batched uneven queue (gr_queue_*), through the C-ABI.
 
 The queue's images must be bitwise those of a plain per-dataset plan
-(gr_recon_device with default settings): the tuned knobs -- slice pairs per
-launch and the column-FFT width -- change no sum order.  The graph is replayed
-several times with eager work of the program between replays, and every replay
-is checked against the eager result.
+(gr_recon_device with default settings) wherever the tuner kept the default
+FFT radix: slice pairs per launch, the column-FFT width and the gridding
+workgroup size change no sum order.  Where it chose radix 4 the FFTs add in
+another order and the images agree to FP32 rounding (rel-L2 <= 1e-6).  The
+graph is replayed several times with eager work of the program between
+replays, and every replay is checked against the eager result.
 """
 import numpy as np
 import pytest
@@ -50,16 +52,25 @@ def plain_results(sets):
     return [r.cpu() for r in res]
 
 
-def test_queue_matches_plain_plans_bitwise_and_replays():
+def same_as_plain(q, d, out, ref):
+    """Bitwise where the tuned radix is the plain default, else FP32 rounding."""
+    o = out.cpu()
+    if q.settings(d)["fft_radix"] == 16:
+        return torch.equal(o, ref)
+    return bool(torch.isfinite(o).all()) and float((o - ref).norm() / ref.norm()) <= 1e-6
+
+
+def test_queue_matches_plain_plans_and_replays():
     sets = uneven_batch(11, 10)
     ref = plain_results(sets)
     q = gr.Queue(sets, n_streams=3, autotune=True)
     torch.cuda.synchronize()
     info = q.info
-    print("queue info", info, flush=True)
+    print("queue info", info, [q.settings(d) for d in range(10)], flush=True)
     assert info["n_datasets"] == 10 and info["slices"] == sum(s[1].shape[0] for s in sets)
     for d, (_, _, out) in enumerate(sets):  # create ran the batch once
-        assert torch.equal(out.cpu(), ref[d]), d
+        assert same_as_plain(q, d, out, ref[d]), d
+    first = [out.cpu() for _, _, out in sets]
     for rep in range(3):
         for _, _, out in sets:
             out.fill_(float("nan"))
@@ -68,16 +79,17 @@ def test_queue_matches_plain_plans_bitwise_and_replays():
         torch.cuda.synchronize()
         assert torch.isfinite(scratch)
         for d, (_, _, out) in enumerate(sets):
-            assert torch.equal(out.cpu(), ref[d]), (rep, d)
+            assert torch.equal(out.cpu(), first[d]), (rep, d)  # every replay bitwise the first run
     for _, _, out in sets:
         out.fill_(float("nan"))
     q.run_sequential()
     torch.cuda.synchronize()
     for d, (_, _, out) in enumerate(sets):
-        assert torch.equal(out.cpu(), ref[d]), ("sequential", d)
+        assert torch.equal(out.cpu(), first[d]), ("sequential", d)
     for d in range(10):
         s = q.settings(d)
         assert s["max_pairs"] in (1, 2, 4, 8) and 0 <= s["stream"] < 3
+        assert s["fft_radix"] in (4, 16) and s["grid_wpb"] in (1, 2, 4)
     q.close()
 
 
@@ -90,6 +102,7 @@ def test_queue_without_autotune_and_single_stream():
     q.run()
     torch.cuda.synchronize()
     for d, (_, _, out) in enumerate(sets):
+        assert q.settings(d)["fft_radix"] == 16
         assert torch.equal(out.cpu(), ref[d]), d
     q.close()
 
