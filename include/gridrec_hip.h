@@ -20,6 +20,7 @@
  *                       normalisation and sub-pixel shift of tomo_recon.cpp:1877,
  *                       3708-3780 (or find_shift.cpp:1685-1697) fused in
  *   gr_center_entropy <- the entropy sweep of find_shift.cpp:1727-1905
+ *   gr_ring_correct  <- RingCorrectionSinogram, tomo_recon.cpp:3905-3972
  *   gr_plan_destroy  <- recon_free        grid.cpp:418-434
  *
  * All functions return 0 on success and a negative GR_E* code on failure;
@@ -196,6 +197,15 @@ int gr_recon_device_raw(gr_plan *plan, const gr_preproc *pre, float *out, long o
 int gr_center_entropy(gr_plan *plan, const float *sino, long sino_slice_stride, long sino_row_stride,
                       const float *shifts, int n_shifts, float hist_min, float hist_max, int n_bins,
                       float *entropy, void *stream);
+
+/* Ring-artefact correction of n_slices float sinograms in place, the step
+ * tomo_recon runs between normalisation and gridrec (RingCorrectionSinogram,
+ * tomo_recon.cpp:3905-3972; the reference runs 20 iterations with
+ * RING_COEFF): per iteration each row is rescaled to the global mean row
+ * level, and the column means minus their 3-tap low pass, times ring_coeff,
+ * are subtracted (results clamped at 0).  Asynchronous on `stream`. */
+int gr_ring_correct(float *sino, long slice_stride, long row_stride, int n_slices, int n_ang, int n_det,
+                    float ring_coeff, int iterations, void *stream);
 
 /* recon_free. */
 int gr_plan_destroy(gr_plan *plan);

@@ -114,6 +114,8 @@ def lib():
         _lib.or_preprocess.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ushort), fp, fp, fp, C.c_float, fp]
         _lib.or_entropy.argtypes = [fp, C.c_long, C.c_float, C.c_float, C.c_int]
         _lib.or_entropy.restype = C.c_float
+        _lib.or_ring_correct.argtypes = [fp, C.c_int, C.c_int, C.c_float, C.c_int]
+        _lib.or_ring_correct.restype = None
         _lib.or_fft_f32.argtypes = [fp, C.c_long, C.c_int]
         _lib.or_fft_f64.argtypes = [dp, C.c_long, C.c_int]
         _lib.or_recon_batch_f32.argtypes = [P, C.c_int, fp, C.c_int, C.c_long, C.c_long, fp, C.c_int]
@@ -243,6 +245,13 @@ def entropy(data: np.ndarray, vmin: float, vmax: float, n_bins: int) -> float:
     """Histogram entropy in bits (find_shift.cpp:2078-2111)."""
     d = np.ascontiguousarray(data, dtype=np.float32).ravel()
     return float(lib().or_entropy(_fptr(d), d.size, vmin, vmax, n_bins))
+
+
+def ring_correct(sino: np.ndarray, ring_coeff: float, iterations: int = 20) -> np.ndarray:
+    """RingCorrectionSinogram (tomo_recon.cpp:3905-3972) on a copy of one sinogram."""
+    out = np.array(sino, dtype=np.float32, order="C", copy=True)
+    lib().or_ring_correct(_fptr(out), out.shape[0], out.shape[1], ring_coeff, iterations)
+    return out
 
 
 def pad_half(rows: np.ndarray) -> np.ndarray:

@@ -820,6 +820,43 @@ float or_entropy(const float *data, long n, float vmin, float vmax, int n_bins) 
     return ent;
 }
 
+void or_ring_correct(float *data, int n_ang, int n_det, float ring_coeff, int iterations) {
+    float *mv = (float *)malloc(sizeof(float) * (size_t)n_ang);
+    float *ml = (float *)malloc(sizeof(float) * (size_t)n_det);
+    float *lp = (float *)malloc(sizeof(float) * (size_t)n_det);
+    if (!mv || !ml || !lp) { free(mv); free(ml); free(lp); return; }
+    for (int m = 0; m < iterations; m++) {
+        float mt = 0.0f;
+        for (int i = 0; i < n_ang; i++) {
+            mv[i] = 0.0f;
+            for (int j = 0; j < n_det; j++) mv[i] += data[(size_t)i * n_det + j];
+            mv[i] /= n_det;
+            mt += mv[i];
+        }
+        mt /= n_ang;
+        for (int i = 0; i < n_ang; i++)
+            for (int j = 0; j < n_det; j++)
+                if (mv[i] != 0.0f) data[(size_t)i * n_det + j] = data[(size_t)i * n_det + j] * mt / mv[i];
+        for (int j = 0; j < n_det; j++) ml[j] = 0.0f;
+        for (int i = 0; i < n_ang; i++)
+            for (int j = 0; j < n_det; j++) ml[j] += data[(size_t)i * n_det + j];
+        for (int j = 0; j < n_det; j++) ml[j] /= n_ang;
+        for (int j = 1; j < n_det - 1; j++) lp[j] = (float)((ml[j - 1] + ml[j] + ml[j + 1]) / 3.0);
+        lp[0] = ml[0];
+        lp[n_det - 1] = ml[n_det - 1];
+        for (int i = 0; i < n_ang; i++)
+            for (int j = 0; j < n_det; j++) {
+                const float t = ml[j] - lp[j];
+                float *d = &data[(size_t)i * n_det + j];
+                if ((*d - (t * ring_coeff)) > 0.0f) *d -= (t * ring_coeff);
+                else *d = 0.0f;
+            }
+    }
+    free(mv);
+    free(ml);
+    free(lp);
+}
+
 void or_pad_half(int n_ang, int n_det, const float *src, float *dst) {
     int off = n_det / 2, w = 2 * n_det;                   /* tomo_recon.cpp:3692,3698 */
     for (int a = 0; a < n_ang; a++) {

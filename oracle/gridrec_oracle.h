@@ -107,6 +107,11 @@ int or_preprocess(int mode, int n_ang, int n_det, const unsigned short *raw, con
  * into a float. */
 float or_entropy(const float *data, long n, float vmin, float vmax, int n_bins);
 
+/* RingCorrectionSinogram (tomo_recon.cpp:3905-3972) on one n_ang x n_det
+ * sinogram in place, `iterations` rounds (the reference: 20), in the
+ * reference's float arithmetic and summation order. */
+void or_ring_correct(float *data, int n_ang, int n_det, float ring_coeff, int iterations);
+
 /* Batch driver used for the CPU baseline: n_pairs slice pairs of the raw
  * (unpadded) sinograms sino[2*pair + {0,1}][n_ang][n_det]; when pad_half is set
  * each pair is padded with or_pad_half and reconstructed with center + n_det/2;

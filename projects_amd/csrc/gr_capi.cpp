@@ -497,6 +497,27 @@ int gr_recon_device(gr_plan *pl, const float *sino, long sino_slice_stride, long
                         n_slices, stream);
 }
 
+int gr_ring_correct(float *sino, long slice_stride, long row_stride, int n_slices, int n_ang, int n_det,
+                    float ring_coeff, int iterations, void *stream) {
+    if (!sino) return fail(GR_EINVAL, "null argument");
+    if (n_slices < 1 || n_ang < 1 || n_det < 1 || iterations < 0 || iterations > 1000 || n_ang > 65535)
+        return fail(GR_EINVAL, "invalid ring-correction sizes");
+    if (row_stride < n_det || slice_stride < row_stride * (long)n_ang)
+        return fail(GR_EINVAL, "sinogram strides smaller than the rows they hold");
+    if (!(fabsf(ring_coeff) < 1e30f)) return fail(GR_EINVAL, "ring coefficient not finite");
+    if (iterations == 0) return GR_OK;
+    hipStream_t s = (hipStream_t)stream;
+    float *rm = nullptr, *cm = nullptr;
+    hipError_t e = hipMallocAsync((void **)&rm, sizeof(float) * (size_t)n_slices * n_ang, s);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&cm, sizeof(float) * (size_t)n_slices * n_det, s);
+    if (e == hipSuccess)
+        e = gr_launch_ring(sino, slice_stride, row_stride, n_slices, n_ang, n_det, ring_coeff, iterations, rm, cm, s);
+    if (rm) (void)hipFreeAsync(rm, s);
+    if (cm) (void)hipFreeAsync(cm, s);
+    if (e != hipSuccess) return hip_fail(e, "ring correction");
+    return GR_OK;
+}
+
 int gr_center_entropy(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride,
                       const float *shifts, int n_shifts, float hist_min, float hist_max, int n_bins, float *entropy,
                       void *stream) {

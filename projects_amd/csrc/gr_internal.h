@@ -110,3 +110,8 @@ hipError_t gr_launch_phantom(float *sino, long slice_stride, int n_slices, long 
 // apart); hist receives n_cand x (n_bins + 1) counts.
 hipError_t gr_launch_hist(const float *img, long cand_stride, long slice_stride, long row_stride, int on,
                           int n_cand, float vmin, float vmax, int n_bins, unsigned *hist, hipStream_t s);
+
+// Ring-artefact correction in place (tomo_recon.cpp:3905-3972); row_mean and
+// col_mean are n_slices x n_ang and n_slices x n_det scratch floats.
+hipError_t gr_launch_ring(float *sino, long slice_stride, long row_stride, int n_slices, int n_ang, int n_det,
+                          float coeff, int iterations, float *row_mean, float *col_mean, hipStream_t s);

@@ -95,6 +95,8 @@ _SIGS = {
                                       C.c_void_p]),
     "gr_center_entropy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.POINTER(C.c_float), C.c_int,
                                     C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float), C.c_void_p]),
+    "gr_ring_correct": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                  C.c_void_p]),
     "gr_plan_destroy": (C.c_int, [C.c_void_p]),
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
@@ -371,6 +373,18 @@ class Queue:
             self.close()
         except Exception:
             pass
+
+
+def ring_correct(sino, ring_coeff: float, iterations: int = 20, stream=None):
+    """Ring-artefact correction in place of a cuda float32 [n_slices, n_ang, n_det]
+    tensor (gr_ring_correct, tomo_recon.cpp:3905-3972)."""
+    import torch
+
+    assert sino.is_cuda and sino.dtype == torch.float32 and sino.dim() == 3 and sino.stride(2) == 1
+    ns, na, nd = sino.shape
+    _check(lib().gr_ring_correct(C.c_void_p(sino.data_ptr()), sino.stride(0), sino.stride(1), ns, na, nd, ring_coeff,
+                                 iterations, C.c_void_p(_stream_handle(stream))), "gr_ring_correct")
+    return sino
 
 
 def phantom_sinograms(out, seed: int, n_ang: int, n_det: int, centers, n_ell: int, photons: float = 0.0,

@@ -81,3 +81,22 @@ def test_entropy_closed_forms():
     # into bin 0, as (int) does: three singleton bins out of five values
     x = np.array([0.5, 2.0, -5.0, 1.0, -0.05], np.float32)
     assert abs(O.entropy(x, 0.0, 1.0, 10) - 3 * (-0.2 * np.log2(0.2))) < 1e-6
+
+
+def test_ring_correction_closed_forms():
+    """RingCorrectionSinogram restatement: a sinogram of identical linear rows
+    has no ring and is left as is; one bright column on a flat background is
+    spread over its neighbours as the 3-tap low pass prescribes; values pushed
+    below zero are clamped."""
+    n_ang, n_det = 30, 40
+    ramp = np.tile(1.0 + 0.01 * np.arange(n_det, dtype=np.float32), (n_ang, 1))
+    assert np.max(np.abs(O.ring_correct(ramp, 1.0) - ramp)) < 1e-5
+    B, delta, k = 2.0, 0.3, 17
+    s = np.full((n_ang, n_det), B, np.float32)
+    s[:, k] += delta
+    one = O.ring_correct(s, 1.0, iterations=1)
+    expect = np.full(n_det, B, np.float64)
+    expect[k - 1:k + 2] = B + delta / 3
+    assert np.max(np.abs(one[0] - expect)) < 1e-5 and np.allclose(one, one[0][None, :])
+    clamp = O.ring_correct(s, 100.0, iterations=1)
+    assert clamp[0, k] == 0.0 and clamp[0, k - 1] > B and clamp[0, k + 1] > B
