@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--n-det", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--accum", default="compensated", choices=["compensated", "plain"])
+    ap.add_argument("--grid-op", default="auto", choices=["auto", "off", "on"],
+                    help="gridding from the precomputed gather operator (plain workload)")
     ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -486,7 +488,8 @@ def main():
     torch.cuda.synchronize()
     gen_s = time.time() - t0
 
-    plan = gr.Plan(n_ang=nang, n_det=ndet, center=center, pad="half", crop=(ndet // 2, ndet), accum=a.accum)
+    plan = gr.Plan(n_ang=nang, n_det=ndet, center=center, pad="half", crop=(ndet // 2, ndet), accum=a.accum,
+                   grid_op=a.grid_op)
     inf = plan.info
     out_n = plan.out_n
     out = torch.empty((ns, out_n, out_n), dtype=torch.float32, device=dev)
@@ -507,6 +510,8 @@ def main():
     dt = time.perf_counter() - t0
     tim = plan.timing(reset=True)
     plan.set_timing(False)
+    inf = plan.info  # after the warm-up, which builds the gather operator in automatic mode
+    grid_op_used = "on" if inf["grid_op_bytes"] > 0 else "off"
     slices_s = ns * a.steps / dt
 
     # roofline of the dominant kernel (events recorded in the timed region)
@@ -532,7 +537,7 @@ def main():
     if os.path.exists(tf) and (nang, ndet, inf["max_pairs"], a.accum) == (1500, 2048, 8, "compensated"):
         with open(tf) as f:
             k = json.load(f)["kernels"].get(dom)
-        if k and k["pairs_per_launch"] == 8:
+        if k and k["pairs_per_launch"] == 8 and k.get("grid_op", "off") == grid_op_used:
             roofline["traffic"] = k["total_bytes"]
             roofline["traffic_read_write"] = [k["read_bytes"], k["write_bytes"]]
             roofline["traffic_source"] = "profiles/gridrec_plain_1500x2048_hbm_traffic.md"
@@ -549,7 +554,9 @@ def main():
                                "PSWF C=6, shlo, 2048^2 window of 4095^2",
                    "slices": ns, "n_ang": nang, "n_det": ndet, "center": center, "pdim": inf["pdim"], "M": M,
                    "M0": inf["M0"], "out_n": out_n, "accum": a.accum, "tables": "reference",
-                   "pairs_per_launch": inf["max_pairs"]},
+                   "pairs_per_launch": inf["max_pairs"], "grid_op": grid_op_used,
+                   "grid_op_MB": round(inf["grid_op_bytes"] / 1e6, 1),
+                   "grid_op_build_ms": round(inf["grid_op_build_ms"], 1)},
         "roofline": roofline,
         "kernels": per_kernel,
         "pipeline_alg_GBs": round(pipeline_gbs, 1),

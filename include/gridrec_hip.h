@@ -58,7 +58,8 @@ enum {
 
 /* Accumulation of the gridding taps into each H cell. */
 enum {
-    GR_ACCUM_COMPENSATED = 0,  /* error-free 2Sum per tap, sum + compensation at the end */
+    GR_ACCUM_COMPENSATED = 0,  /* fused multiply-add partials of a few sample lines, added with an
+                                  error-free 2Sum; sum + compensation at the end */
     GR_ACCUM_PLAIN = 1         /* plain float adds, as grid.cpp:297-300 */
 };
 
@@ -66,6 +67,17 @@ enum {
 enum {
     GR_TABLES_REFERENCE = 0,   /* bit-identical to recon_init's float tables (host) */
     GR_TABLES_FP64 = 1         /* evaluated in FP64 on the GPU, then rounded to float */
+};
+
+/* The gridding step's precomputed gather operator: the list of every grid
+ * cell's taps (sample, weight), built once per plan on the GPU, after which
+ * each group of slice pairs streams it instead of evaluating the geometry.
+ * Images are bitwise the same with or without it. */
+enum {
+    GR_GRIDOP_AUTO = 0,  /* built on the first call that brings the plan's total to 16 pairs, when
+                            its size is at most a third of the free device memory */
+    GR_GRIDOP_OFF = 1,   /* never built */
+    GR_GRIDOP_ON = 2     /* built at plan creation; creation fails if it cannot be */
 };
 
 /* PSWF record, layout of pswf_struct (grid.h:37-44). */
@@ -99,6 +111,7 @@ typedef struct gr_params {
     int tables;            /* GR_TABLES_* */
     int device;            /* HIP device ordinal */
     int max_pairs;         /* slice pairs per launch (workspace sizing); 0 = automatic */
+    int grid_op;           /* GR_GRIDOP_* */
     /* optional overrides, as recon_init honours grid_struct.filter and
        grid_struct.pswf (grid.cpp:78-81): a filter function evaluated on the
        host for the filter table (only with GR_TABLES_REFERENCE), and a PSWF
@@ -116,6 +129,8 @@ typedef struct gr_info {
     long out_off, out_n;    /* output window */
     int max_pairs;
     size_t device_bytes;    /* device memory held by the plan */
+    size_t grid_op_bytes;   /* of which the gather operator (0 while not built) */
+    double grid_op_build_ms; /* host time its build took */
 } gr_info;
 
 /* Per-kernel timing of the last gr_recon_device call(s) (ms, summed over

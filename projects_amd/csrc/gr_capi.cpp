@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -119,10 +120,79 @@ int plan_stream(gr_plan *pl, hipStream_t *s) {
     return GR_OK;
 }
 
+// Tiles with more rows than this are gridded one pair per wave: a few dozen
+// near the centre of the grid, whose thousands of rows would otherwise make
+// the launch wait for their waves.
+constexpr unsigned kGopHeavyRows = 64;
+
+// Build K2's gather operator on stream s: count pass, exclusive sum of the
+// per-tile entry counts on the host, fill pass.  Blocks the host until it is
+// built.  With `required` false a geometry whose operator would not fit (its
+// sample index range, its entry count, or a third of the free device memory)
+// is declined quietly and the plan keeps evaluating the geometry per launch.
+int build_gop(gr_plan *pl, hipStream_t s, bool required) {
+    const auto t0 = std::chrono::steady_clock::now();
+    const GrGeom &g = pl->g;
+    const int n_ang = pl->prm.n_ang;
+    auto decline = [&](const char *why) {
+        pl->gop_state = -1;
+        return required ? fail(GR_ENOMEM, std::string("gather operator: ") + why) : GR_OK;
+    };
+    pl->gop_state = -1;  // until it is built: a failed build is not tried again
+    if (!gr_gop_fits(g, n_ang)) return decline("sample index range too large");
+    const int nt = pl->ntiles;
+    GR_HIP(hipMalloc((void **)&pl->d_gop_n, sizeof(unsigned) * nt));
+    GR_HIP(hipMalloc((void **)&pl->d_gop_off, sizeof(unsigned) * nt));
+    pl->dev_bytes += 2 * sizeof(unsigned) * nt;
+    GrDevTables t = pl->tables();
+    // count pass: rows of both layouts per tile (the aligned counts go to the
+    // offsets array, overwritten below)
+    GR_HIP(gr_launch_gop_count(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, s));
+    std::vector<unsigned> cnt(nt), cnt_al(nt), off(nt);
+    GR_HIP(hipMemcpyAsync(cnt.data(), pl->d_gop_n, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s));
+    GR_HIP(hipMemcpyAsync(cnt_al.data(), pl->d_gop_off, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s));
+    GR_HIP(hipStreamSynchronize(s));
+    // per tile the aligned layout (wave-uniform merges) where it costs at
+    // most a quarter more rows than the ragged one
+    unsigned long long total = 0;
+    for (int i = 0; i < nt; i++) {
+        if (cnt[i] >= kGopAlignedBit || cnt_al[i] >= kGopAlignedBit) return decline("tile row count overflow");
+        if (cnt_al[i] && 4ull * cnt_al[i] <= 5ull * cnt[i]) {
+            cnt[i] = cnt_al[i] | kGopAlignedBit;
+            pl->gop_aligned_tiles++;
+        }
+        // the prefix of the list (nearest to the centre first) that ends with
+        // the last tile of more than kGopHeavyRows rows takes the
+        // one-pair-per-wave path
+        if ((cnt[i] & ~kGopAlignedBit) > kGopHeavyRows) pl->gop_heavy = i + 1;
+        off[i] = (unsigned)total;
+        total += 64ull * (cnt[i] & ~kGopAlignedBit);
+        if (total > 0xffffffffull) return decline("more than 2^32 entries");
+    }
+    GR_HIP(hipMemcpyAsync(pl->d_gop_n, cnt.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
+    const size_t bytes = sizeof(uint2) * (size_t)std::max<unsigned long long>(total, 1);
+    size_t free_b = 0, total_b = 0;
+    if (!required && (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b / 3))
+        return decline("larger than a third of the free device memory");
+    if (hipMalloc((void **)&pl->d_gop, bytes) != hipSuccess) {
+        pl->d_gop = nullptr;
+        (void)hipGetLastError();
+        return decline("device allocation failed");
+    }
+    pl->dev_bytes += bytes;
+    pl->gop_bytes = bytes + 2 * sizeof(unsigned) * nt;
+    GR_HIP(hipMemcpyAsync(pl->d_gop_off, off.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
+    GR_HIP(gr_launch_gop_fill(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, pl->d_gop, s));
+    GR_HIP(hipStreamSynchronize(s));
+    pl->gop_state = 1;
+    pl->gop_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return GR_OK;
+}
+
 void free_plan(gr_plan *pl) {
     if (!pl) return;
-    void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase,
-                    pl->d_akey, pl->d_aord, pl->d_ang, pl->d_in, pl->d_out};
+    void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
+                    pl->d_aord, pl->d_ang, pl->d_in, pl->d_out, pl->d_gop, pl->d_gop_n, pl->d_gop_off};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -398,6 +468,12 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
     } else {
         pl->owns_workspace = false;
     }
+    if (params->grid_op < GR_GRIDOP_AUTO || params->grid_op > GR_GRIDOP_ON) CK(fail(GR_EINVAL, "unknown grid_op"));
+    if (params->grid_op == GR_GRIDOP_ON) {
+        hipStream_t bs;
+        CK(plan_stream(pl, &bs));
+        CK(build_gop(pl, bs, true));
+    }
 #undef CK
     *plan_out = pl;
     return GR_OK;
@@ -421,6 +497,8 @@ int gr_plan_info(const gr_plan *pl, gr_info *info) {
     info->out_n = g.out_n;
     info->max_pairs = pl->max_pairs;
     info->device_bytes = pl->dev_bytes;
+    info->grid_op_bytes = pl->gop_bytes;
+    info->grid_op_build_ms = pl->gop_build_ms;
     return GR_OK;
 }
 
@@ -440,6 +518,11 @@ int gr_plan_tables(const gr_plan *pl, float *sine, float *cose, float *filphase,
 
 namespace {
 
+// Pairs after which a plan in GR_GRIDOP_AUTO builds its gather operator: two
+// full launches of the widest gridding kernel, about where the build (two
+// geometry passes) has paid for itself.
+constexpr long kGopAutoPairs = 16;
+
 // The chunk loop of gr_recon_device / gr_recon_device_raw: K1..K4 per chunk
 // of max_pairs pairs; K1 reads float rows (pre null) or detector counts.
 int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long sino_row_stride, const GrPre *pre,
@@ -450,8 +533,18 @@ int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long si
         return fail(GR_EINVAL, "image strides smaller than the window they hold");
     GR_HIP(hipSetDevice(pl->prm.device));
     hipStream_t s = (hipStream_t)stream;  // NULL: the null stream
-    const GrDevTables t = pl->tables();
     const int n_pairs = (n_slices + 1) / 2;
+    if (pl->gop_state == 0 && pl->prm.grid_op == GR_GRIDOP_AUTO && pl->pairs_done + n_pairs >= kGopAutoPairs) {
+        // not while a graph is being captured: the build synchronises
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GR_HIP(hipStreamIsCapturing(s, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            const int rc = build_gop(pl, s, false);
+            if (rc) return rc;
+        }
+    }
+    pl->pairs_done += n_pairs;
+    const GrDevTables t = pl->tables();
     for (int p0 = 0; p0 < n_pairs; p0 += pl->max_pairs) {
         const int np = std::min(pl->max_pairs, n_pairs - p0);
         {

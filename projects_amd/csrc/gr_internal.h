@@ -49,6 +49,11 @@ struct GrDevTables {
     const float4 *ang;             // n_ang in sorted order: COSE, SINE, 1/COSE, 1/SINE (0 if |.| <= 1e-6)
     const unsigned *tiles;         // tile list (tx | ty << 16), nearest-to-centre first
     int ntiles;
+    // precomputed gather operator of K2 (null when the plan has none):
+    // entries, taps per tile, first entry of each tile
+    const uint2 *gop = nullptr;
+    const unsigned *gop_n = nullptr, *gop_off = nullptr;
+    int gop_heavy = 0;  // leading tiles of the list gridded one pair per wave
 };
 
 // ---- kernel launchers (one per .hip file) ----------------------------------
@@ -76,9 +81,23 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
 // K2: tile-local gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
 // The plan lists the kernel's tiles (gr_grid_tile_size() cells per side).
 int gr_grid_tile_size();
+// With t.gop set, K2 streams the precomputed operator instead of evaluating
+// the geometry; the images are bitwise the same.
 hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
                           const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
                           int accum, hipStream_t s);
+
+// The precomputed operator: whether the geometry's sample indices fit its
+// entries; the count pass (rows per tile of the ragged layout into tile_n,
+// of the aligned layout into tile_na); the fill pass (entries at
+// tile_off[tile]; tile_n then holds the chosen rows, with kGopAlignedBit set
+// for the aligned layout, and tile_off the exclusive sum of 64 * rows).
+constexpr unsigned kGopAlignedBit = 1u << 31;
+bool gr_gop_fits(const GrGeom &g, int n_ang);
+hipError_t gr_launch_gop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
+                               unsigned *tile_na, hipStream_t s);
+hipError_t gr_launch_gop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
+                              const unsigned *tile_off, uint2 *gop, hipStream_t s);
 
 // K3: FFT (-1) of each row of Ht (over a), output rows of the window only:
 // Z[pairs][M][out_n].

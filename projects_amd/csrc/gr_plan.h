@@ -19,6 +19,17 @@ struct gr_plan {
     float4 *d_ang = nullptr;
     unsigned *d_tiles = nullptr;
     int ntiles = 0;
+    // K2's precomputed gather operator (GR_GRIDOP_*): entries, taps per tile,
+    // first entry per tile; gop_state 0 = not built yet, 1 = built,
+    // -1 = declined (too large), not tried again
+    uint2 *d_gop = nullptr;
+    unsigned *d_gop_n = nullptr, *d_gop_off = nullptr;
+    size_t gop_bytes = 0;
+    double gop_build_ms = 0.0;
+    int gop_state = 0;
+    int gop_aligned_tiles = 0;  // tiles in the aligned layout
+    int gop_heavy = 0;          // leading tiles gridded one pair per wave
+    long pairs_done = 0;  // pairs reconstructed so far (the automatic build's trigger)
     // workspace for max_pairs pairs: owned, or a region bound by a queue
     int max_pairs = 1;
     bool owns_workspace = true;
@@ -53,6 +64,12 @@ struct gr_plan {
         t.ang = d_ang;
         t.tiles = d_tiles;
         t.ntiles = ntiles;
+        if (gop_state == 1) {
+            t.gop = d_gop;
+            t.gop_n = d_gop_n;
+            t.gop_off = d_gop_off;
+            t.gop_heavy = gop_heavy;
+        }
         return t;
     }
 };

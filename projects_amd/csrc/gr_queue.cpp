@@ -121,6 +121,10 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     for (int d = 0; d < n_datasets; d++) {
         gr_params p = datasets[d].params;
         p.device = device;
+        // a queue replays captured launches, where the automatic operator build
+        // never runs (it synchronises); it would only disturb the tuner's
+        // timings, so datasets build one only when asked to (GR_GRIDOP_ON)
+        if (p.grid_op == GR_GRIDOP_AUTO) p.grid_op = GR_GRIDOP_OFF;
         if (datasets[d].n_slices < 1 || !datasets[d].sino || !datasets[d].out) {
             free_queue(q);
             return gr_set_error(GR_EINVAL, "dataset without slices or buffers");

@@ -113,6 +113,64 @@ __device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int 
     __builtin_amdgcn_wave_barrier();
 }
 
+// Walk the sample lines that can reach the wave's tile, in the kernel's fixed
+// order (sorted angle keys, the window's two ranges in turn; per line the
+// direct samples, then the mirrored ones, in chunks of at most 64 ascending
+// j), calling on_chunk(mirror, row_off, c0, nj, j0, j1, C, S) per chunk --
+// [c0, c0+nj) the tile's samples of the chunk, [j0, j1] the lane's own
+// candidates among them -- and on_line() after each line that passed the
+// tile tests.  Tiles beyond the largest sample radius visit nothing.
+template <class ChunkFn, class LineFn>
+__device__ __forceinline__ void walk_tile(const GridArgs &A, int tx, int ty, bool valid, float du, float dv,
+                                          ChunkFn &&on_chunk, LineFn &&on_line) {
+    const float M2 = A.g.fM2;
+    const float cu = (float)(tx * kTile) + 3.5f - M2, cv = (float)(ty * kTile) + 3.5f - M2;
+    if (!(sqrtf(cu * cu + cv * cv) <= A.rmax)) return;
+    const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
+    int r0, r1, r2, r3;
+    gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
+    for (int side = 0; side < 2; side++) {
+        const int k0 = side ? r2 : r0, k1 = side ? r3 : r1;
+        for (int k = k0; k < k1; k++) {
+            const float4 cs = A.ang[k];
+            // does the line pass through the tile grown by the box half-width?
+            if (fabsf(cu * cs.y - cv * cs.x) > reach * (fabsf(cs.x) + fabsf(cs.y)) + 0.05f) continue;
+            int td0, td1, tm0, tm1;
+            gr_candidates_r(cu, cv, cs.z, cs.w, reach, A.g, td0, td1, tm0, tm1);
+            if (td0 > td1 && tm0 > tm1) continue;
+            int jd0, jd1, jm0, jm1;
+            gr_candidates_r(du, dv, cs.z, cs.w, A.L2m, A.g, jd0, jd1, jm0, jm1);
+            if (!valid) {
+                jd1 = jd0 - 1;
+                jm1 = jm0 - 1;
+            }
+            const long row_off = (long)A.aord[k] * A.half;
+            for (int c0 = td0; c0 <= td1; c0 += 64)
+                on_chunk(0, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0), min(min(jd1, td1), c0 + 63), cs.x,
+                         cs.y);
+            for (int c0 = tm0; c0 <= tm1; c0 += 64)
+                on_chunk(1, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0), min(min(jm1, tm1), c0 + 63), cs.x,
+                         cs.y);
+            on_line();
+        }
+    }
+}
+
+template <int ACCUM, int P>
+__device__ __forceinline__ void store_cells(const GridArgs &A, int pair0, int nvalid, int a, int b,
+                                            const float4 (&acc)[P]) {
+    const int M = A.g.M;
+    if (a < M && b < M) {
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            if (p >= nvalid) break;
+            const float4 s = acc[p];
+            const float2 h = ACCUM == GR_ACCUM_COMPENSATED ? make_float2(s.x + s.y, s.z + s.w) : make_float2(s.x, s.z);
+            A.ht[((long)(pair0 + p) * M + b) * M + a] = h;
+        }
+    }
+}
+
 template <int ACCUM, int P>
 __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     extern __shared__ float smem[];
@@ -131,81 +189,295 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
     const int M = A.g.M;
     const float M2 = A.g.fM2;
-    const float cu = (float)(tx * kTile) + 3.5f - M2, cv = (float)(ty * kTile) + 3.5f - M2;
 
     const bool valid = a >= 1 && b >= 1 && a < M && b < M;
     const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
-    const float du = pa - M2, dv = pb - M2;
-    const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
     float4 acc[P];   // (sum re, error re, sum im, error im)
-    float2 part[P];  // compensated mode: this angle's partial
+    float2 part[P];  // compensated mode: the open partial
 #pragma unroll
     for (int p = 0; p < P; p++) {
         acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         part[p] = make_float2(0.0f, 0.0f);
     }
 
-    if (sqrtf(cu * cu + cv * cv) <= A.rmax) {
-        int lines = 0;  // sample lines in the open partial
-        int r0, r1, r2, r3;
-        gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
-        const long pair_stride = (long)A.n_ang * A.half;
-        const float2 *base1 = A.spec1 + pair0 * pair_stride;
-        const float2 *base2 = A.spec2 + pair0 * pair_stride;
-        for (int side = 0; side < 2; side++) {
-            const int k0 = side ? r2 : r0, k1 = side ? r3 : r1;
-            for (int k = k0; k < k1; k++) {
-                const float4 cs = A.ang[k];
-                // does the line pass through the tile grown by the box half-width?
-                if (fabsf(cu * cs.y - cv * cs.x) > reach * (fabsf(cs.x) + fabsf(cs.y)) + 0.05f) continue;
-                int td0, td1, tm0, tm1;
-                gr_candidates_r(cu, cv, cs.z, cs.w, reach, A.g, td0, td1, tm0, tm1);
-                if (td0 > td1 && tm0 > tm1) continue;
-                int jd0, jd1, jm0, jm1;
-                gr_candidates_r(du, dv, cs.z, cs.w, A.L2m, A.g, jd0, jd1, jm0, jm1);
-                if (!valid) {
-                    jd1 = jd0 - 1;
-                    jm1 = jm0 - 1;
-                }
-                const long row_off = (long)A.aord[k] * A.half;
-                for (int c0 = td0; c0 <= td1; c0 += 64)
-                    chunk<ACCUM, P>(base1, pair_stride, nvalid, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0),
-                                    min(min(jd1, td1), c0 + 63), cs.x, cs.y, pa, pb, A.g, s_wtbl, stage, lane, acc,
-                                    part);
-                for (int c0 = tm0; c0 <= tm1; c0 += 64)
-                    chunk<ACCUM, P>(base2, pair_stride, nvalid, row_off, c0, min(64, tm1 - c0 + 1), max(jm0, c0),
-                                    min(min(jm1, tm1), c0 + 63), cs.x, cs.y, ma, mb, A.g, s_wtbl, stage, lane, acc,
-                                    part);
-                if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
-                    merge<P>(acc, part);
-                    lines = 0;
+    int lines = 0;  // sample lines in the open partial
+    const long pair_stride = (long)A.n_ang * A.half;
+    const float2 *base1 = A.spec1 + pair0 * pair_stride;
+    const float2 *base2 = A.spec2 + pair0 * pair_stride;
+    walk_tile(
+        A, tx, ty, valid, pa - M2, pb - M2,
+        [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
+            chunk<ACCUM, P>(mirror ? base2 : base1, pair_stride, nvalid, row_off, c0, nj, j0, j1, C, S,
+                            mirror ? ma : pa, mirror ? mb : pb, A.g, s_wtbl, stage, lane, acc, part);
+        },
+        [&]() {
+            if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
+                merge<P>(acc, part);
+                lines = 0;
+            }
+        });
+    if (ACCUM == GR_ACCUM_COMPENSATED && lines) merge<P>(acc, part);
+    store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
+}
+
+// ---- the precomputed gather operator ----------------------------------------
+//
+// The taps of a cell -- which sample, direct or mirrored, and its weight --
+// depend only on the plan.  A plan that reconstructs many slices lists them
+// once (k_gop_build) and then grids every group of pairs by streaming the
+// list (k_grid_op) instead of re-evaluating the geometry.
+//
+// Layout: per tile (wavefront) n_t rows of 64 entries, entry k of lane l at
+// gop[off_t + 64 k + l] (one coalesced 512-byte row per k).  Entry =
+// (idx | mirror << 30 | merge << 31, weight bits): idx = angle*half + j
+// indexes the pair's spec1 (mirror 0) or spec2 (mirror 1); merge asks for
+// k_grid's merge of the compensated partial after this entry.  A lane's taps
+// come in k_grid's order, so k_grid_op performs, per cell, the same float
+// operations in the same order: its images are bitwise k_grid's.  Padding
+// entries are (0, 0) or (merge, 0): sample 0 of the pair's first row, which K1
+// always writes as zero, times weight 0 -- adding it changes no bit, and
+// merging a zero partial changes no bit either.
+//
+// Two layouts, chosen per tile by the host from the count pass:
+// * ragged: each lane's taps back to back, n_t = the largest count in the
+//   wave, the merge mark on the lane's last tap before each merge point;
+// * aligned: the taps between two merge points (kMergeAngles sample lines)
+//   occupy the same rows in every lane, padded to the longest lane, with the
+//   merge mark on the group's last row in every lane -- so the merge is
+//   wave-uniform.  Near the centre of the grid every line crosses the tile
+//   and the lanes' groups are nearly equal: there it costs few extra rows and
+//   turns a merge in almost every row (some lane always closes a group) into
+//   one per group.
+constexpr unsigned kGopMirror = 1u << 30, kGopMerge = 1u << 31, kGopIdx = kGopMirror - 1;
+constexpr unsigned kGopAligned = kGopAlignedBit, kGopRows = kGopAligned - 1;  // tile_n: layout bit, rows
+
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, o));
+    return v;
+}
+
+// MODE 0 (count): rows of the ragged layout into tile_n, of the aligned one
+// into tile_na.  MODE 1 (fill): the entries at tile_off in the layout that
+// tile_n's top bit names (its rows in the low bits).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n, unsigned *tile_na,
+                                                   const unsigned *tile_off, uint2 *gop) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (ti >= A.ntiles) return;
+    const unsigned tt = A.tiles[ti];
+    const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
+    const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+    const int M = A.g.M;
+    const float M2 = A.g.fM2;
+    const bool valid = a >= 1 && b >= 1 && a < M && b < M;
+    const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
+    const unsigned long base = MODE == 1 ? (unsigned long)tile_off[ti] + lane : 0;
+    const bool aligned = MODE == 1 && (tile_n[ti] & kGopAligned);
+
+    unsigned n = 0;   // ragged: taps of this lane so far
+    unsigned g0 = 0;  // aligned: first row of the open group
+    unsigned ng = 0;  // taps of this lane in the open group
+    unsigned na = 0;  // count pass: rows of the aligned layout
+    uint2 pend = make_uint2(0u, 0u);
+    bool has_pend = false;  // the lane's latest tap, not yet written (it may get the merge mark)
+    auto put = [&](unsigned row, uint2 e) { gop[base + 64ul * row] = e; };
+    auto close_group = [&]() {  // a merge point of k_grid (wave-uniform)
+        if (MODE == 0) {
+            na += wave_max(ng);
+        } else if (aligned) {
+            const unsigned len = wave_max(ng);
+            if (has_pend) put(g0 + ng - 1, ng == len ? make_uint2(pend.x | kGopMerge, pend.y) : pend);
+            for (unsigned r = ng; r < len; r++) put(g0 + r, make_uint2(r + 1 == len ? kGopMerge : 0u, 0u));
+            g0 += len;
+        } else if (has_pend) {
+            put(n - 1, make_uint2(pend.x | kGopMerge, pend.y));
+        }
+        has_pend = false;
+        ng = 0;
+    };
+    int lines = 0;
+    walk_tile(
+        A, tx, ty, valid, pa - M2, pb - M2,
+        [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
+            for (int j = j0; j <= j1; j++) {
+                float w;
+                if (gr_tap(j, C, S, mirror ? ma : pa, mirror ? mb : pb, A.g, A.wtbl, w)) {
+                    if (MODE == 1 && has_pend) put(aligned ? g0 + ng - 1 : n - 1, pend);
+                    pend = make_uint2((unsigned)(row_off + j) | (mirror ? kGopMirror : 0u), __float_as_uint(w));
+                    has_pend = true;
+                    n++;
+                    ng++;
                 }
             }
+        },
+        [&]() {
+            if (++lines == kMergeAngles) {
+                lines = 0;
+                close_group();
+            }
+        });
+    if (lines) close_group();  // k_grid's final merge
+    if (MODE == 0) {
+        n = wave_max(n);
+        if (lane == 0) {
+            tile_n[ti] = n;
+            tile_na[ti] = na;
         }
-        if (ACCUM == GR_ACCUM_COMPENSATED && lines) merge<P>(acc, part);
-    }
-    if (a < M && b < M) {
-#pragma unroll
-        for (int p = 0; p < P; p++) {
-            if (p >= nvalid) break;
-            const float4 s = acc[p];
-            const float2 h = ACCUM == GR_ACCUM_COMPENSATED ? make_float2(s.x + s.y, s.z + s.w) : make_float2(s.x, s.z);
-            A.ht[((long)(pair0 + p) * M + b) * M + a] = h;
-        }
+    } else {
+        const unsigned rows = tile_n[ti] & kGopRows;
+        for (unsigned r = aligned ? g0 : n; r < rows; r++) put(r, make_uint2(0u, 0u));
     }
 }
 
+struct GopArgs {
+    const uint2 *gop;
+    const unsigned *tile_n, *tile_off;
+    int n_heavy;  // tiles [0, n_heavy) of the list take the one-pair-per-wave path
+};
+
+__device__ __forceinline__ uint2 gop_load(const uint2 *p) {
+    const unsigned long long u = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+    return make_uint2((unsigned)u, (unsigned)(u >> 32));
+}
+
+// The P pairs' samples of one entry.
+template <int P>
+__device__ __forceinline__ void gop_gather(uint2 t, const float2 *base1, const float2 *base2, long pair_stride,
+                                           int nvalid, float2 (&v)[P]) {
+    const float2 *src = ((t.x & kGopMirror) ? base2 : base1) + (t.x & kGopIdx);
+#pragma unroll
+    for (int p = 0; p < P; p++) v[p] = src[(long)(p < nvalid ? p : nvalid - 1) * pair_stride];
+}
+
+// k_grid's accumulation of one tap (and its merge, when marked).
 template <int ACCUM, int P>
-hipError_t launch(GridArgs A, int groups, int ntiles, size_t lds_base, int wpb, hipStream_t s) {
+__device__ __forceinline__ void gop_consume(uint2 t, const float2 (&v)[P], float4 (&acc)[P], float2 (&part)[P]) {
+    const float w = __uint_as_float(t.y);
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        if (ACCUM == GR_ACCUM_COMPENSATED) {
+            part[p].x = __builtin_fmaf(w, v[p].x, part[p].x);
+            part[p].y = __builtin_fmaf(w, v[p].y, part[p].y);
+        } else {
+            acc[p].x += w * v[p].x;
+            acc[p].z += w * v[p].y;
+        }
+    }
+    if (ACCUM == GR_ACCUM_COMPENSATED && (t.x & kGopMerge)) merge<P>(acc, part);
+}
+
+constexpr int kHeavyBatch = 16;  // rows loaded at once on the one-pair path
+
+// The one-pair-per-wave path for the few tiles near the centre of the grid
+// whose cells take thousands of taps: the P pairs of a group go to P
+// wavefronts, which stream the tile's rows kHeavyBatch at a time (all entry
+// loads, then all sample loads, then the adds), so each wave has many loads in
+// flight instead of a chain of dependent ones.  Per cell and pair the float
+// operations are those of the wide path.
+template <int ACCUM>
+__device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &O, int ti, int pair, int lane) {
+    const unsigned tt = A.tiles[ti];
+    const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
+    const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+    const long pair_stride = (long)A.n_ang * A.half;
+    const float2 *base1 = A.spec1 + pair * pair_stride;
+    const float2 *base2 = A.spec2 + pair * pair_stride;
+    const uint2 *e = O.gop + (unsigned long)O.tile_off[ti] + lane;
+    const unsigned n = O.tile_n[ti] & kGopRows;
+    float4 acc[1] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    float2 part[1] = {make_float2(0.0f, 0.0f)};
+    for (unsigned k0 = 0; k0 < n; k0 += kHeavyBatch) {
+        uint2 t[kHeavyBatch];
+        float2 v[kHeavyBatch][1];
+#pragma unroll
+        for (int i = 0; i < kHeavyBatch; i++) {
+            t[i] = gop_load(e + 64ul * min(k0 + i, n - 1));
+            if (k0 + i >= n) t[i] = make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (int i = 0; i < kHeavyBatch; i++) gop_gather<1>(t[i], base1, base2, pair_stride, 1, v[i]);
+#pragma unroll
+        for (int i = 0; i < kHeavyBatch; i++) gop_consume<ACCUM, 1>(t[i], v[i], acc, part);
+    }
+    store_cells<ACCUM, 1>(A, pair, 1, a, b, acc);
+}
+
+// k_grid's accumulation, fed by the operator's entries.  Blocks
+// [0, heavy_blocks) take the tiles [0, n_heavy) one pair per wave; the other
+// blocks take one tile per wave for the P pairs of the group, software-
+// pipelined: entries are loaded four rows ahead, the samples of row k+1 are
+// gathered while row k is added.  Rows past a tile's end are zero entries.
+template <int ACCUM, int P>
+__global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int pair0 = A.pair_base + blockIdx.y * P;
+    const int nvalid = min(P, A.pair_end - pair0);
+    const int heavy_blocks = (O.n_heavy * P + 3) / 4;
+    if ((int)blockIdx.x < heavy_blocks) {
+        const int hw = blockIdx.x * 4 + wave;
+        const int ti = hw / P, p = hw % P;
+        if (ti < O.n_heavy && p < nvalid) grid_op_heavy<ACCUM>(A, O, ti, pair0 + p, lane);
+        return;
+    }
+    const int ti = O.n_heavy + (blockIdx.x - heavy_blocks) * 4 + wave;
+    if (ti >= A.ntiles) return;
+    const unsigned tt = A.tiles[ti];
+    const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
+    const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+
+    float4 acc[P];
+    float2 part[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+        acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        part[p] = make_float2(0.0f, 0.0f);
+    }
+    const long pair_stride = (long)A.n_ang * A.half;
+    const float2 *base1 = A.spec1 + pair0 * pair_stride;
+    const float2 *base2 = A.spec2 + pair0 * pair_stride;
+    const uint2 *e = O.gop + (unsigned long)O.tile_off[ti] + lane;
+    const unsigned n = O.tile_n[ti] & kGopRows;
+    if (n) {
+        const uint2 zero = make_uint2(0u, 0u);
+        auto row = [&](unsigned k) { return gop_load(e + 64ul * min(k, n - 1)); };
+        uint2 t0 = row(0), t1 = row(1), t2 = row(2), t3 = row(3);
+        float2 va[P], vb[P];
+        gop_gather<P>(t0, base1, base2, pair_stride, nvalid, va);
+        for (unsigned k = 0; k < n; k += 2) {
+            // rows k (t0, samples in va) and k+1 (t1); t2, t3 hold rows k+2, k+3
+            const uint2 c1 = k + 1 < n ? t1 : zero;
+            gop_gather<P>(c1, base1, base2, pair_stride, nvalid, vb);
+            const uint2 c0 = t0;
+            t0 = t2;
+            t2 = row(k + 4);
+            gop_consume<ACCUM, P>(c0, va, acc, part);
+            gop_gather<P>(k + 2 < n ? t0 : zero, base1, base2, pair_stride, nvalid, va);
+            t1 = t3;
+            t3 = row(k + 5);
+            gop_consume<ACCUM, P>(c1, vb, acc, part);
+        }
+    }
+    store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
+}
+
+template <int ACCUM, int P>
+hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_base, int wpb, hipStream_t s) {
+    if (O) {
+        const int blocks = (O->n_heavy * P + 3) / 4 + (A.ntiles - O->n_heavy + 3) / 4;
+        hipLaunchKernelGGL((k_grid_op<ACCUM, P>), dim3(blocks, groups), dim3(256), 0, s, A, *O);
+        return hipGetLastError();
+    }
     const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
     hipError_t e = gr_allow_lds(k_grid<ACCUM, P>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3((ntiles + wpb - 1) / wpb, groups), dim3(64 * wpb), lds, s, A);
+    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3((A.ntiles + wpb - 1) / wpb, groups), dim3(64 * wpb), lds, s, A);
     return hipGetLastError();
 }
 
 template <int ACCUM>
-hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, int wpb, hipStream_t s) {
+hipError_t launch_all(GridArgs A, const GopArgs *O, int pairs, size_t lds_base, int wpb, hipStream_t s) {
     // groups of 8 pairs; a remainder of 5-7 rides in one more (partial) group
     // of 8, one of 3-4 in a group of 4, 2 and 1 in their own widths: a tap's
     // geometry costs the same for any width, so one partial wide group beats
@@ -214,25 +486,18 @@ hipError_t launch_all(GridArgs A, int pairs, int ntiles, size_t lds_base, int wp
     hipError_t e = hipSuccess;
     A.pair_end = pairs;
     A.pair_base = 0;
-    if (full + (rem >= 5) > 0) e = launch<ACCUM, 8>(A, full + (rem >= 5), ntiles, lds_base, wpb, s);
+    if (full + (rem >= 5) > 0) e = launch<ACCUM, 8>(A, O, full + (rem >= 5), lds_base, wpb, s);
     if (e != hipSuccess || rem == 0 || rem >= 5) return e;
     A.pair_base = full * 8;
-    if (rem >= 3) return launch<ACCUM, 4>(A, 1, ntiles, lds_base, wpb, s);
-    if (rem == 2) return launch<ACCUM, 2>(A, 1, ntiles, lds_base, wpb, s);
-    return launch<ACCUM, 1>(A, 1, ntiles, lds_base, wpb, s);
+    if (rem >= 3) return launch<ACCUM, 4>(A, O, 1, lds_base, wpb, s);
+    if (rem == 2) return launch<ACCUM, 2>(A, O, 1, lds_base, wpb, s);
+    return launch<ACCUM, 1>(A, O, 1, lds_base, wpb, s);
 }
 
-}  // namespace
-
-int gr_grid_tile_size() { return kTile; }
-
-hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
-                          const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
-                          int accum, hipStream_t s) {
+GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) {
     GridArgs A;
-    A.spec1 = spec1;
-    A.spec2 = spec2;
-    A.ht = ht;
+    A.spec1 = A.spec2 = nullptr;
+    A.ht = nullptr;
     A.ang = t.ang;
     A.aord = t.aord;
     A.akey = t.akey;
@@ -243,6 +508,7 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     A.half = (int)g.half;
     A.lds_wtbl = (int)((ltbl + 1 + 1) & ~1L);  // keep the staging area 8-byte aligned
     A.pair_base = 0;
+    A.pair_end = 0;
     A.g.scale = g.scale;
     A.g.L2 = g.L2;
     A.g.tblspcg = g.tblspcg;
@@ -254,8 +520,42 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     A.dmax = 3.5f * 1.41421356f + A.L2m * 1.41421356f + 0.05f;
     // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
     A.rmax = g.scale * (float)(g.half - 1) * 1.0001f + A.dmax + 0.5f;
+    return A;
+}
+
+}  // namespace
+
+int gr_grid_tile_size() { return kTile; }
+
+hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
+                          const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
+                          int accum, hipStream_t s) {
+    GridArgs A = make_args(g, t, n_ang, ltbl);
+    A.spec1 = spec1;
+    A.spec2 = spec2;
+    A.ht = ht;
+    GopArgs O{t.gop, t.gop_n, t.gop_off, t.gop_heavy};
+    const GopArgs *op = t.gop ? &O : nullptr;
     const size_t lds_base = sizeof(float) * A.lds_wtbl;
     const int wpb = g.grid_wpb == 1 || g.grid_wpb == 2 ? g.grid_wpb : 4;
-    if (accum == GR_ACCUM_PLAIN) return launch_all<GR_ACCUM_PLAIN>(A, pairs, t.ntiles, lds_base, wpb, s);
-    return launch_all<GR_ACCUM_COMPENSATED>(A, pairs, t.ntiles, lds_base, wpb, s);
+    if (accum == GR_ACCUM_PLAIN) return launch_all<GR_ACCUM_PLAIN>(A, op, pairs, lds_base, wpb, s);
+    return launch_all<GR_ACCUM_COMPENSATED>(A, op, pairs, lds_base, wpb, s);
+}
+
+bool gr_gop_fits(const GrGeom &g, int n_ang) { return (long)n_ang * g.half <= (long)kGopIdx + 1; }
+
+hipError_t gr_launch_gop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
+                               unsigned *tile_na, hipStream_t s) {
+    const GridArgs A = make_args(g, t, n_ang, ltbl);
+    hipLaunchKernelGGL(k_gop_build<0>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, tile_n, tile_na,
+                       (const unsigned *)nullptr, (uint2 *)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_gop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
+                              const unsigned *tile_off, uint2 *gop, hipStream_t s) {
+    const GridArgs A = make_args(g, t, n_ang, ltbl);
+    hipLaunchKernelGGL(k_gop_build<1>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, tile_n, (unsigned *)nullptr,
+                       tile_off, gop);
+    return hipGetLastError();
 }

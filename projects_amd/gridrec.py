@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(_HERE, "libgridrec_hip.so")
 GR_OK, GR_EINVAL, GR_ENOMEM, GR_EHIP, GR_ENOPSWF, GR_ENODEV = 0, -1, -2, -3, -4, -5
 PAD = {"none": 0, "half": 1}
 ACCUM = {"compensated": 0, "plain": 1}
+GRID_OP = {"auto": 0, "off": 1, "on": 2}
 TABLES = {"reference": 0, "fp64": 1}
 
 
@@ -36,7 +37,7 @@ class GrParams(C.Structure):
         ("R", C.c_float), ("X0", C.c_float), ("Y0", C.c_float), ("filter", C.c_char * 16),
         ("ltbl", C.c_long), ("pad", C.c_int), ("crop_off", C.c_long), ("crop_n", C.c_long),
         ("accum", C.c_int), ("tables", C.c_int), ("device", C.c_int), ("max_pairs", C.c_int),
-        ("filter_fn", C.c_void_p), ("pswf", C.POINTER(GrPswf)),
+        ("grid_op", C.c_int), ("filter_fn", C.c_void_p), ("pswf", C.POINTER(GrPswf)),
     ]
 
 
@@ -45,6 +46,7 @@ class GrInfo(C.Structure):
         ("pdim", C.c_long), ("M", C.c_long), ("M0", C.c_long), ("M02", C.c_long), ("sampl", C.c_float),
         ("L", C.c_float), ("scale", C.c_float), ("n_det_eff", C.c_int), ("center_eff", C.c_float),
         ("out_off", C.c_long), ("out_n", C.c_long), ("max_pairs", C.c_int), ("device_bytes", C.c_size_t),
+        ("grid_op_bytes", C.c_size_t), ("grid_op_build_ms", C.c_double),
     ]
 
 
@@ -184,7 +186,7 @@ def make_params(n_ang: int, n_det: int, center: float, geom: int = 1, angles: Op
                 filter: str = "shlo", C_: float = 6.0, sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0,
                 X0: float = 0.0, Y0: float = 0.0, ltbl: int = 512, pad: str = "none",
                 crop: Optional[Tuple[int, int]] = None, accum: str = "compensated", tables: str = "reference",
-                device: int = 0, max_pairs: int = 0):
+                device: int = 0, max_pairs: int = 0, grid_op: str = "auto"):
     """gr_params of one geometry, and the angle array it points to (keep it alive)."""
     ang = None
     ang_ptr = None
@@ -193,7 +195,8 @@ def make_params(n_ang: int, n_det: int, center: float, geom: int = 1, angles: Op
         ang_ptr = ang.ctypes.data_as(C.POINTER(C.c_float))
     crop_off, crop_n = crop if crop is not None else (0, 0)
     prm = GrParams(n_ang, n_det, geom, ang_ptr, center, C_, sampl, MaxPixSiz, R, X0, Y0, filter.encode()[:15], ltbl,
-                   PAD[pad], crop_off, crop_n, ACCUM[accum], TABLES[tables], device, max_pairs, None, None)
+                   PAD[pad], crop_off, crop_n, ACCUM[accum], TABLES[tables], device, max_pairs,
+                   GRID_OP[grid_op], None, None)
     return prm, ang
 
 
@@ -204,16 +207,23 @@ class Plan:
                  angles: Optional[Sequence[float]] = None, filter: str = "shlo", C_: float = 6.0,
                  sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0, X0: float = 0.0, Y0: float = 0.0,
                  ltbl: int = 512, pad: str = "none", crop: Optional[Tuple[int, int]] = None,
-                 accum: str = "compensated", tables: str = "reference", device: int = 0, max_pairs: int = 0):
+                 accum: str = "compensated", tables: str = "reference", device: int = 0, max_pairs: int = 0,
+                 grid_op: str = "auto"):
         self.params, self._angles = make_params(n_ang, n_det, center, geom, angles, filter, C_, sampl, MaxPixSiz, R,
-                                                X0, Y0, ltbl, pad, crop, accum, tables, device, max_pairs)
+                                                X0, Y0, ltbl, pad, crop, accum, tables, device, max_pairs, grid_op)
         h = C.c_void_p()
         _check(lib().gr_plan_create(C.byref(self.params), C.byref(h)), "gr_plan_create")
         self._h = h
         self.n_ang, self.n_det, self.device = n_ang, n_det, device
+        self._out_n = int(self.info["out_n"])
+
+    @property
+    def info(self) -> dict:
+        """gr_plan_info as a dict (device_bytes and the gather operator's
+        fields change when the plan builds its operator)."""
         inf = GrInfo()
         _check(lib().gr_plan_info(self._h, C.byref(inf)), "gr_plan_info")
-        self.info = {f: getattr(inf, f) for f, _ in GrInfo._fields_}
+        return {f: getattr(inf, f) for f, _ in GrInfo._fields_}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -228,7 +238,7 @@ class Plan:
 
     @property
     def out_n(self) -> int:
-        return int(self.info["out_n"])
+        return self._out_n
 
     def tables(self) -> dict:
         inf = self.info

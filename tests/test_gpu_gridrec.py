@@ -163,19 +163,21 @@ def test_batch_odd_count_strided_and_pairing():
     p.close()
 
 
+@pytest.mark.parametrize("grid_op", ["off", "on"])
 @pytest.mark.parametrize("accum", ["compensated", "plain"])
-def test_pair_grouping_is_bitwise_neutral(accum):
+def test_pair_grouping_is_bitwise_neutral(accum, grid_op):
     """Pairs per launch 1..16 group the gridding into launches of 8, 4, 2, 1
     pairs and partial groups of 8 (5-7 pairs) or 4 (3 pairs); every grouping
-    gives bitwise the images of one pair per launch."""
+    gives bitwise the images of one pair per launch, with the geometry
+    evaluated per launch or streamed from the precomputed operator."""
     g = O.Geometry(n_ang=72, n_det=96, center=47.25)
     rng = np.random.default_rng(21)
     ns = 31  # 16 pairs, the last one odd
     sino = torch.from_numpy(rng.standard_normal((ns, 72, 96)).astype(np.float32)).cuda()
-    ref = plan_for(g, max_pairs=1, accum=accum).recon(sino)
+    ref = plan_for(g, max_pairs=1, accum=accum, grid_op=grid_op).recon(sino)
     torch.cuda.synchronize()
     for mp in (2, 3, 4, 5, 6, 7, 8, 13, 16):
-        out = plan_for(g, max_pairs=mp, accum=accum).recon(sino)
+        out = plan_for(g, max_pairs=mp, accum=accum, grid_op=grid_op).recon(sino)
         torch.cuda.synchronize()
         assert torch.equal(out, ref), mp
 
