@@ -71,6 +71,7 @@ int dev_alloc(gr_plan *pl, T **dst, size_t n) {
 struct SharedTable {
     void *ptr;
     int refs;
+    size_t count;  // elements
 };
 std::mutex g_shared_mu;
 std::map<std::tuple<int, int, long>, SharedTable> g_shared;
@@ -90,11 +91,9 @@ int shared_table(int device, int kind, long n, Build build, T **out, int *count)
             if (d) (void)hipFree(d);
             return fail(GR_ENOMEM, std::string("shared table: ") + hipGetErrorString(e));
         }
-        it = g_shared.emplace(key, SharedTable{d, 0}).first;
-        if (count) *count = (int)h.size();
-    } else if (count) {
-        *count = -1;  // caller already knows the size
+        it = g_shared.emplace(key, SharedTable{d, 0, h.size()}).first;
     }
+    if (count) *count = (int)it->second.count;
     it->second.refs++;
     *out = (T *)it->second.ptr;
     return GR_OK;
@@ -161,10 +160,10 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required) {
             cnt[i] = cnt_al[i] | kGopAlignedBit;
             pl->gop_aligned_tiles++;
         }
-        // the prefix of the list (nearest to the centre first) that ends with
-        // the last tile of more than kGopHeavyRows rows takes the
+        // the leading slots of every sector (nearest to the centre first) up to
+        // the last tile of more than kGopHeavyRows rows take the
         // one-pair-per-wave path
-        if ((cnt[i] & ~kGopAlignedBit) > kGopHeavyRows) pl->gop_heavy = i + 1;
+        if ((cnt[i] & ~kGopAlignedBit) > kGopHeavyRows) pl->gop_heavy = std::max(pl->gop_heavy, i % pl->sector_len + 1);
         off[i] = (unsigned)total;
         total += 64ull * (cnt[i] & ~kGopAlignedBit);
         if (total > 0xffffffffull) return decline("more than 2^32 entries");
@@ -187,6 +186,37 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required) {
     pl->gop_state = 1;
     pl->gop_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return GR_OK;
+}
+
+// The gridding tile list: kGridSectors sectors of the polar angle of the
+// tile centre (seen from the grid centre), each ordered by the squared
+// distance of the tile centre from the grid centre (exact integers in units of
+// 1/4 cell) and then row-major, padded with kGridNoTile to the longest
+// sector's length.  Entries are tx | ty << 16.
+std::vector<unsigned> sector_tiles(int nt, int ts, long M) {
+    const long c2 = 2 * (M >> 1), h2 = ts - 1;
+    std::vector<std::vector<unsigned long long>> sec(kGridSectors);
+    for (int ty = 0; ty < nt; ty++)
+        for (int tx = 0; tx < nt; tx++) {
+            const long x = 2L * tx * ts + h2 - c2, y = 2L * ty * ts + h2 - c2;
+            double phi = atan2((double)y, (double)x);
+            if (phi < 0) phi += 2 * M_PI;
+            const int sx = std::min(kGridSectors - 1, (int)(phi / (2 * M_PI / kGridSectors)));
+            const size_t i = (size_t)ty * nt + tx;
+            sec[sx].push_back(((unsigned long long)(x * x + y * y) << 24) | i);
+        }
+    size_t len = 0;
+    for (auto &v : sec) {
+        std::sort(v.begin(), v.end());
+        len = std::max(len, v.size());
+    }
+    std::vector<unsigned> out(kGridSectors * len, kGridNoTile);
+    for (int sx = 0; sx < kGridSectors; sx++)
+        for (size_t k = 0; k < sec[sx].size(); k++) {
+            const unsigned idx = (unsigned)(sec[sx][k] & 0xffffff);
+            out[sx * len + k] = (idx % (unsigned)nt) | ((idx / (unsigned)nt) << 16);
+        }
+    return out;
 }
 
 void free_plan(gr_plan *pl) {
@@ -417,34 +447,21 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         }
         CK(dev_upload(pl, &pl->d_ang, ang.data(), n_ang));
     }
-    // tiles of the gridding kernel over the M x M grid, nearest to the centre
-    // first (shared per size)
+    // tiles of the gridding kernel over the M x M grid (shared per size):
+    // eight sectors of the polar angle of the tile centre, each listed nearest
+    // to the centre first and padded to a common length with kNoTile; the
+    // kernels give sector x to the workgroups that the hardware places on
+    // XCD x, so each XCD's L2 holds the samples of its own sector
     {
         const int ts = gr_grid_tile_size();
         const int nt = (int)((g.M + ts - 1) / ts);
         if ((long)nt * nt >= (1L << 24)) CK(fail(GR_EINVAL, "grid too large for the tile list"));
-        auto build = [&] {
-            // key: squared distance of the tile centre from the grid centre, in
-            // units of 1/4 cell (exact integers), then the row-major index
-            const long c2 = 2 * (g.M >> 1), h2 = ts - 1;
-            std::vector<unsigned long long> key((size_t)nt * nt);
-            for (int ty = 0; ty < nt; ty++)
-                for (int tx = 0; tx < nt; tx++) {
-                    const long x = 2L * tx * ts + h2 - c2, y = 2L * ty * ts + h2 - c2;
-                    const size_t i = (size_t)ty * nt + tx;
-                    key[i] = ((unsigned long long)(x * x + y * y) << 24) | i;
-                }
-            std::sort(key.begin(), key.end());
-            std::vector<unsigned> sorted(key.size());
-            for (size_t i = 0; i < key.size(); i++) {
-                const unsigned idx = (unsigned)(key[i] & 0xffffff);
-                sorted[i] = (idx % (unsigned)nt) | ((idx / (unsigned)nt) << 16);
-            }
-            return sorted;
-        };
-        CK(shared_table(params->device, kTiles, g.M, build, &pl->d_tiles, nullptr));
-        pl->ntiles = nt * nt;
-        pl->dev_bytes += sizeof(unsigned) * (size_t)nt * nt;
+        int slots = 0;
+        CK(shared_table(params->device, kTiles, g.M, [&] { return sector_tiles(nt, ts, g.M); }, &pl->d_tiles,
+                        &slots));
+        pl->sector_len = slots / kGridSectors;
+        pl->ntiles = slots;
+        pl->dev_bytes += sizeof(unsigned) * (size_t)pl->ntiles;
     }
     // workspace
     const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;

@@ -47,13 +47,14 @@ struct GrDevTables {
     const float *akey;             // n_ang angle keys in [0, pi), sorted
     const int *aord;               // n_ang angle index of each sorted key
     const float4 *ang;             // n_ang in sorted order: COSE, SINE, 1/COSE, 1/SINE (0 if |.| <= 1e-6)
-    const unsigned *tiles;         // tile list (tx | ty << 16), nearest-to-centre first
-    int ntiles;
+    const unsigned *tiles;         // tile list (tx | ty << 16): kGridSectors sectors of sector_len
+                                   // slots, nearest to the centre first, kGridNoTile padding
+    int ntiles, sector_len;        // ntiles = kGridSectors * sector_len
     // precomputed gather operator of K2 (null when the plan has none):
     // entries, taps per tile, first entry of each tile
     const uint2 *gop = nullptr;
     const unsigned *gop_n = nullptr, *gop_off = nullptr;
-    int gop_heavy = 0;  // leading tiles of the list gridded one pair per wave
+    int gop_heavy = 0;  // leading slots of every sector gridded one pair per wave
 };
 
 // ---- kernel launchers (one per .hip file) ----------------------------------
@@ -81,6 +82,10 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
 // K2: tile-local gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
 // The plan lists the kernel's tiles (gr_grid_tile_size() cells per side).
 int gr_grid_tile_size();
+// Sectors of the tile list: one per XCD of the MI355X (workgroup b runs on
+// XCD b mod 8), and the padding value of a sector's unused slots.
+constexpr int kGridSectors = 8;
+constexpr unsigned kGridNoTile = 0xffffffffu;
 // With t.gop set, K2 streams the precomputed operator instead of evaluating
 // the geometry; the images are bitwise the same.
 hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,

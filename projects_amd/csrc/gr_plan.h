@@ -18,7 +18,8 @@ struct gr_plan {
     int *d_aord = nullptr;
     float4 *d_ang = nullptr;
     unsigned *d_tiles = nullptr;
-    int ntiles = 0;
+    int ntiles = 0;      // slots: kGridSectors * sector_len
+    int sector_len = 0;
     // K2's precomputed gather operator (GR_GRIDOP_*): entries, taps per tile,
     // first entry per tile; gop_state 0 = not built yet, 1 = built,
     // -1 = declined (too large), not tried again
@@ -28,7 +29,7 @@ struct gr_plan {
     double gop_build_ms = 0.0;
     int gop_state = 0;
     int gop_aligned_tiles = 0;  // tiles in the aligned layout
-    int gop_heavy = 0;          // leading tiles gridded one pair per wave
+    int gop_heavy = 0;          // leading slots of each sector gridded one pair per wave
     long pairs_done = 0;  // pairs reconstructed so far (the automatic build's trigger)
     // workspace for max_pairs pairs: owned, or a region bound by a queue
     int max_pairs = 1;
@@ -64,6 +65,7 @@ struct gr_plan {
         t.ang = d_ang;
         t.tiles = d_tiles;
         t.ntiles = ntiles;
+        t.sector_len = sector_len;
         if (gop_state == 1) {
             t.gop = d_gop;
             t.gop_n = d_gop_n;

@@ -57,10 +57,19 @@ struct GridArgs {
     const float *akey;   // sorted keys
     const unsigned *tiles;
     const float *wtbl;
-    int ntiles, n_ang, half, lds_wtbl, pair_base, pair_end;
+    int ntiles, sector_len, n_ang, half, lds_wtbl, pair_base, pair_end;
     GrGatherGeom g;
     float L2m, dmax, rmax;
 };
+
+// Slot of the tile list for wave `wave` of workgroup `block` in a launch of
+// `wpb` waves per workgroup: workgroup b runs on XCD b mod 8 and takes sector
+// b mod 8 of the list, its slots first + (b / 8) * wpb + wave in order.
+// Returns -1 past the sector's end.
+__device__ __forceinline__ int sector_slot(const GridArgs &A, int block, int wpb, int wave, int first) {
+    const int slot = first + (block / kGridSectors) * wpb + wave;
+    return slot < A.sector_len ? (block % kGridSectors) * A.sector_len + slot : -1;
+}
 
 // s + c += x with s the float sum and c the carried rounding errors.
 __device__ __forceinline__ void two_sum_add(float &s, float &c, float x) {
@@ -179,12 +188,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
-    if (ti >= A.ntiles) return;
+    const int ti = sector_slot(A, blockIdx.x, blockDim.x >> 6, wave, 0);
+    if (ti < 0) return;
+    const unsigned tt = A.tiles[ti];
+    if (tt == kGridNoTile) return;
     float2(*stage)[64] = reinterpret_cast<float2(*)[64]>(smem + A.lds_wtbl) + wave * P;
     const int pair0 = A.pair_base + blockIdx.y * P;
     const int nvalid = min(P, A.pair_end - pair0);
-    const unsigned tt = A.tiles[ti];
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
     const int M = A.g.M;
@@ -267,6 +277,10 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
     const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
     if (ti >= A.ntiles) return;
     const unsigned tt = A.tiles[ti];
+    if (tt == kGridNoTile) {  // padding slot: no rows
+        if (MODE == 0 && lane == 0) tile_n[ti] = tile_na[ti] = 0;
+        return;
+    }
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
     const int M = A.g.M;
@@ -334,7 +348,7 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
 struct GopArgs {
     const uint2 *gop;
     const unsigned *tile_n, *tile_off;
-    int n_heavy;  // tiles [0, n_heavy) of the list take the one-pair-per-wave path
+    int n_heavy;  // slots [0, n_heavy) of every sector take the one-pair-per-wave path
 };
 
 __device__ __forceinline__ uint2 gop_load(const uint2 *p) {
@@ -404,26 +418,39 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
     store_cells<ACCUM, 1>(A, pair, 1, a, b, acc);
 }
 
-// k_grid's accumulation, fed by the operator's entries.  Blocks
-// [0, heavy_blocks) take the tiles [0, n_heavy) one pair per wave; the other
-// blocks take one tile per wave for the P pairs of the group, software-
-// pipelined: entries are loaded four rows ahead, the samples of row k+1 are
-// gathered while row k is added.  Rows past a tile's end are zero entries.
+// Workgroups of k_grid_op given to the one-pair-per-wave path (a multiple of
+// the sector count, so the other workgroups keep their XCD-to-sector map).
+template <int P>
+__host__ __device__ __forceinline__ int heavy_blocks(int n_heavy) {
+    return (kGridSectors * n_heavy * P + 4 * kGridSectors - 1) / (4 * kGridSectors) * kGridSectors;
+}
+
+// k_grid's accumulation, fed by the operator's entries.  The first
+// heavy_blocks workgroups take slots [0, n_heavy) of every sector one pair per
+// wave; the others take one tile per wave for the P pairs of the group,
+// software-pipelined: entries are loaded four rows ahead, the samples of row
+// k+1 are gathered while row k is added.  Rows past a tile's end are zero
+// entries.
 template <int ACCUM, int P>
 __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int pair0 = A.pair_base + blockIdx.y * P;
     const int nvalid = min(P, A.pair_end - pair0);
-    const int heavy_blocks = (O.n_heavy * P + 3) / 4;
-    if ((int)blockIdx.x < heavy_blocks) {
+    const int hb = heavy_blocks<P>(O.n_heavy);
+    if ((int)blockIdx.x < hb) {
+        // wave hw: sector hw / (n_heavy P), slot (hw / P) mod n_heavy, pair hw mod P
         const int hw = blockIdx.x * 4 + wave;
-        const int ti = hw / P, p = hw % P;
-        if (ti < O.n_heavy && p < nvalid) grid_op_heavy<ACCUM>(A, O, ti, pair0 + p, lane);
+        const int sx = hw / (O.n_heavy * P), r = hw % (O.n_heavy * P);
+        const int slot = r / P, p = r % P;
+        if (sx >= kGridSectors || p >= nvalid) return;
+        const int ti = sx * A.sector_len + slot;
+        if (A.tiles[ti] != kGridNoTile) grid_op_heavy<ACCUM>(A, O, ti, pair0 + p, lane);
         return;
     }
-    const int ti = O.n_heavy + (blockIdx.x - heavy_blocks) * 4 + wave;
-    if (ti >= A.ntiles) return;
+    const int ti = sector_slot(A, blockIdx.x - hb, 4, wave, O.n_heavy);
+    if (ti < 0) return;
     const unsigned tt = A.tiles[ti];
+    if (tt == kGridNoTile) return;
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
 
@@ -465,14 +492,16 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
 template <int ACCUM, int P>
 hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_base, int wpb, hipStream_t s) {
     if (O) {
-        const int blocks = (O->n_heavy * P + 3) / 4 + (A.ntiles - O->n_heavy + 3) / 4;
+        const int light = (A.sector_len - O->n_heavy + 3) / 4;
+        const int blocks = heavy_blocks<P>(O->n_heavy) + kGridSectors * light;
         hipLaunchKernelGGL((k_grid_op<ACCUM, P>), dim3(blocks, groups), dim3(256), 0, s, A, *O);
         return hipGetLastError();
     }
     const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
     hipError_t e = gr_allow_lds(k_grid<ACCUM, P>, lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3((A.ntiles + wpb - 1) / wpb, groups), dim3(64 * wpb), lds, s, A);
+    const int blocks = kGridSectors * ((A.sector_len + wpb - 1) / wpb);
+    hipLaunchKernelGGL((k_grid<ACCUM, P>), dim3(blocks, groups), dim3(64 * wpb), lds, s, A);
     return hipGetLastError();
 }
 
@@ -504,6 +533,7 @@ GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) 
     A.tiles = t.tiles;
     A.wtbl = t.wtbl;
     A.ntiles = t.ntiles;
+    A.sector_len = t.sector_len;
     A.n_ang = n_ang;
     A.half = (int)g.half;
     A.lds_wtbl = (int)((ltbl + 1 + 1) & ~1L);  // keep the staging area 8-byte aligned
