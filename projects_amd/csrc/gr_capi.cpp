@@ -333,22 +333,25 @@ int gr_filter_lookup(const char *name, char *name_io, int *filter_id) {
 
 int gr_set_error(int code, const char *msg) { return fail(code, msg); }
 
-size_t gr_plan_pair_bytes(const gr_plan *pl) {
+size_t gr_plan_workspace_bytes(const gr_plan *pl, int max_pairs) {
     const GrGeom &g = pl->g;
     const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
+    const size_t spec_pairs = (size_t)(max_pairs + kSpecGroup - 1) / kSpecGroup * kSpecGroup;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    return 2 * up(sizeof(float2) * spec_pp) + up(sizeof(float2) * ht_pp) + up(sizeof(float2) * z_pp);
+    return 2 * up(sizeof(float2) * spec_pp * spec_pairs) + up(sizeof(float2) * ht_pp * max_pairs) +
+           up(sizeof(float2) * z_pp * max_pairs);
 }
 
 void gr_plan_bind_workspace(gr_plan *pl, void *base, int max_pairs) {
     const GrGeom &g = pl->g;
     const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t spec_pairs = (size_t)(max_pairs + kSpecGroup - 1) / kSpecGroup * kSpecGroup;
     char *p = (char *)base;
     pl->d_spec1 = (float2 *)p;
-    p += up(sizeof(float2) * spec_pp * max_pairs);
+    p += up(sizeof(float2) * spec_pp * spec_pairs);
     pl->d_spec2 = (float2 *)p;
-    p += up(sizeof(float2) * spec_pp * max_pairs);
+    p += up(sizeof(float2) * spec_pp * spec_pairs);
     pl->d_ht = (float2 *)p;
     p += up(sizeof(float2) * ht_pp * max_pairs);
     pl->d_z = (float2 *)p;
@@ -478,8 +481,9 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
     }
     pl->max_pairs = mp;
     if (alloc_workspace) {
-        CK(dev_alloc(pl, &pl->d_spec1, spec_pp * mp));
-        CK(dev_alloc(pl, &pl->d_spec2, spec_pp * mp));
+        const size_t spec_pairs = (size_t)(mp + kSpecGroup - 1) / kSpecGroup * kSpecGroup;
+        CK(dev_alloc(pl, &pl->d_spec1, spec_pp * spec_pairs));
+        CK(dev_alloc(pl, &pl->d_spec2, spec_pp * spec_pairs));
         CK(dev_alloc(pl, &pl->d_ht, ht_pp * mp));
         CK(dev_alloc(pl, &pl->d_z, z_pp * mp));
     } else {

@@ -59,8 +59,15 @@ struct GrDevTables {
 
 // ---- kernel launchers (one per .hip file) ----------------------------------
 
-// K1: row FFT (+1), filter/phase, Hermitian split.  spec1/spec2 are
-// [pairs][n_ang][half] complex; pair p reads slices 2*(pair0+p) and +1.
+// The spectra of a chunk's pairs are interleaved by groups of kSpecGroup
+// pairs: pair p's sample j of angle n is element
+// ((p / kSpecGroup) * n_ang + n) * half + j) * kSpecGroup + p % kSpecGroup
+// of spec1 (spec2), so the gridding step reads one tap of all the pairs of a
+// group as one contiguous 64-byte record.
+constexpr int kSpecGroup = 8;
+
+// K1: row FFT (+1), filter/phase, Hermitian split into spec1/spec2 (layout
+// above); pair p of the chunk reads slices 2*(pair0+p) and +1.
 // Detector-count input of K1: flat/dark normalisation and sub-pixel shift
 // applied while loading (mode GR_PRE_FLATDARK or GR_PRE_FLATDARK_LOG).
 struct GrPre {

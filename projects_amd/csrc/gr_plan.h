@@ -79,9 +79,10 @@ struct gr_plan {
 // Create a plan; with alloc_workspace false the plan gets no workspace and
 // must be bound to one (gr_plan_bind_workspace) before it runs.
 int gr_plan_create_impl(const gr_params *params, gr_plan **plan, bool alloc_workspace);
-// Workspace bytes one slice pair needs.
-size_t gr_plan_pair_bytes(const gr_plan *plan);
-// Point the plan's workspace at base (gr_plan_pair_bytes * max_pairs bytes,
-// 256-byte aligned), owned by the caller.
+// Workspace bytes for max_pairs slice pairs per launch (the spectra are held
+// in whole groups of kSpecGroup pairs).
+size_t gr_plan_workspace_bytes(const gr_plan *plan, int max_pairs);
+// Point the plan's workspace at base (gr_plan_workspace_bytes(plan, max_pairs)
+// bytes, 256-byte aligned), owned by the caller.
 void gr_plan_bind_workspace(gr_plan *plan, void *base, int max_pairs);
 int gr_set_error(int code, const char *msg);

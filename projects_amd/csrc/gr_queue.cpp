@@ -159,7 +159,7 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     size_t tune_bytes = 0;
     for (int d = 0; d < n_datasets; d++) {
         const int pairs = (q->ds[d].n_slices + 1) / 2;
-        tune_bytes = std::max(tune_bytes, gr_plan_pair_bytes(q->plans[d]) * (size_t)std::min(pairs, 8));
+        tune_bytes = std::max(tune_bytes, gr_plan_workspace_bytes(q->plans[d], std::min(pairs, 8)));
     }
     void *tune_ws = nullptr;
     if (hipMalloc(&tune_ws, tune_bytes) != hipSuccess) {
@@ -264,7 +264,7 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         q->stream_of[d] = s;
         load[s] += q->cost_ms[d];
-        ws_bytes[s] = std::max(ws_bytes[s], gr_plan_pair_bytes(q->plans[d]) * (size_t)q->plans[d]->max_pairs);
+        ws_bytes[s] = std::max(ws_bytes[s], gr_plan_workspace_bytes(q->plans[d], q->plans[d]->max_pairs));
     }
     q->workspaces.assign(n_streams, nullptr);
     for (int s = 0; s < n_streams; s++) {

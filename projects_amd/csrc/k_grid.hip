@@ -71,6 +71,12 @@ __device__ __forceinline__ int sector_slot(const GridArgs &A, int block, int wpb
     return slot < A.sector_len ? (block % kGridSectors) * A.sector_len + slot : -1;
 }
 
+// Offset of pair `pair`'s first value in the interleaved spectra: its group's
+// block of n_ang * half records, then its slot in the record.
+__device__ __forceinline__ long spec_group_offset(const GridArgs &A, int pair) {
+    return (long)(pair / kSpecGroup) * A.n_ang * A.half * kSpecGroup + pair % kSpecGroup;
+}
+
 // s + c += x with s the float sum and c the carried rounding errors.
 __device__ __forceinline__ void two_sum_add(float &s, float &c, float x) {
     const float t = s + x;
@@ -90,31 +96,61 @@ __device__ __forceinline__ void merge(float4 (&acc)[P], float2 (&part)[P]) {
     }
 }
 
-// Stage samples c0 .. c0+nj-1 of angle n for the P pairs, then add the taps
-// of samples [j0, j1] into cell (qa, qb).  In a partial group (nvalid < P)
-// the spare slots repeat the last valid pair: their accumulators are never
-// stored, and unconditional loads keep the P loads issued back to back.
-template <int ACCUM, int P>
-__device__ __forceinline__ void chunk(const float2 *spec, long pair_stride, int nvalid, long row_off, int c0, int nj, int j0,
-                                      int j1, float C, float S, float qa, float qb, const GrGatherGeom &g,
-                                      const float *wtbl, float2 (*stage)[64], int lane, float4 (&acc)[P],
-                                      float2 (&part)[P]) {
+// The P pairs' values of one sample: P consecutive float2 of its interleaved
+// record (16-byte loads from P = 2 up; the record is 16-byte aligned).
+template <int P>
+__device__ __forceinline__ void load_pairs(const float2 *src, float2 (&v)[P]) {
+    if (P == 1) {
+        v[0] = src[0];
+    } else {
+        const float4 *q = reinterpret_cast<const float4 *>(src);
 #pragma unroll
-    for (int p = 0; p < P; p++)
-        if (lane < nj) stage[p][lane] = spec[(p < nvalid ? p : nvalid - 1) * pair_stride + row_off + c0 + lane];
+        for (int i = 0; i < P / 2; i++) {
+            const float4 t = q[i];
+            v[2 * i] = make_float2(t.x, t.y);
+            v[2 * i + 1] = make_float2(t.z, t.w);
+        }
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void store_pairs(float2 *dst, const float2 (&v)[P]) {
+    if (P == 1) {
+        dst[0] = v[0];
+    } else {
+#pragma unroll
+        for (int i = 0; i < P / 2; i++)
+            reinterpret_cast<float4 *>(dst)[i] = make_float4(v[2 * i].x, v[2 * i].y, v[2 * i + 1].x, v[2 * i + 1].y);
+    }
+}
+
+// Stage samples c0 .. c0+nj-1 of the sample row at row_off (one lane per
+// sample, all P pairs of the group), then add the taps of samples [j0, j1]
+// into cell (qa, qb).  In a partial group the spare slots hold other data:
+// their accumulators are never stored.
+template <int ACCUM, int P>
+__device__ __forceinline__ void chunk(const float2 *spec, long row_off, int c0, int nj, int j0, int j1, float C,
+                                      float S, float qa, float qb, const GrGatherGeom &g, const float *wtbl,
+                                      float2 (*stage)[P], int lane, float4 (&acc)[P], float2 (&part)[P]) {
+    if (lane < nj) {
+        float2 v[P];
+        load_pairs<P>(spec + (row_off + c0 + lane) * kSpecGroup, v);
+        store_pairs<P>(stage[lane], v);
+    }
     __builtin_amdgcn_wave_barrier();
     for (int j = j0; j <= j1; j++) {
         float w;
         if (gr_tap(j, C, S, qa, qb, g, wtbl, w)) {
+            float2 v[P];
+            load_pairs<P>(stage[j - c0], v);
 #pragma unroll
             for (int p = 0; p < P; p++) {
-                const float2 v = stage[p][j - c0];
                 if (ACCUM == GR_ACCUM_COMPENSATED) {
-                    part[p].x = __builtin_fmaf(w, v.x, part[p].x);
-                    part[p].y = __builtin_fmaf(w, v.y, part[p].y);
+                    part[p].x = __builtin_fmaf(w, v[p].x, part[p].x);
+                    part[p].y = __builtin_fmaf(w, v[p].y, part[p].y);
                 } else {
-                    acc[p].x += w * v.x;
-                    acc[p].z += w * v.y;
+                    acc[p].x += w * v[p].x;
+                    acc[p].z += w * v[p].y;
                 }
             }
         }
@@ -192,7 +228,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     if (ti < 0) return;
     const unsigned tt = A.tiles[ti];
     if (tt == kGridNoTile) return;
-    float2(*stage)[64] = reinterpret_cast<float2(*)[64]>(smem + A.lds_wtbl) + wave * P;
+    float2(*stage)[P] = reinterpret_cast<float2(*)[P]>(smem + A.lds_wtbl) + wave * 64;
     const int pair0 = A.pair_base + blockIdx.y * P;
     const int nvalid = min(P, A.pair_end - pair0);
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
@@ -211,14 +247,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     }
 
     int lines = 0;  // sample lines in the open partial
-    const long pair_stride = (long)A.n_ang * A.half;
-    const float2 *base1 = A.spec1 + pair0 * pair_stride;
-    const float2 *base2 = A.spec2 + pair0 * pair_stride;
+    const long group = spec_group_offset(A, pair0);
+    const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
     walk_tile(
         A, tx, ty, valid, pa - M2, pb - M2,
         [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
-            chunk<ACCUM, P>(mirror ? base2 : base1, pair_stride, nvalid, row_off, c0, nj, j0, j1, C, S,
-                            mirror ? ma : pa, mirror ? mb : pb, A.g, s_wtbl, stage, lane, acc, part);
+            chunk<ACCUM, P>(mirror ? base2 : base1, row_off, c0, nj, j0, j1, C, S, mirror ? ma : pa,
+                            mirror ? mb : pb, A.g, s_wtbl, stage, lane, acc, part);
         },
         [&]() {
             if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
@@ -356,13 +391,11 @@ __device__ __forceinline__ uint2 gop_load(const uint2 *p) {
     return make_uint2((unsigned)u, (unsigned)(u >> 32));
 }
 
-// The P pairs' samples of one entry.
+// The P pairs' samples of one entry (spare slots of a partial group read
+// other data; their accumulators are never stored).
 template <int P>
-__device__ __forceinline__ void gop_gather(uint2 t, const float2 *base1, const float2 *base2, long pair_stride,
-                                           int nvalid, float2 (&v)[P]) {
-    const float2 *src = ((t.x & kGopMirror) ? base2 : base1) + (t.x & kGopIdx);
-#pragma unroll
-    for (int p = 0; p < P; p++) v[p] = src[(long)(p < nvalid ? p : nvalid - 1) * pair_stride];
+__device__ __forceinline__ void gop_gather(uint2 t, const float2 *base1, const float2 *base2, float2 (&v)[P]) {
+    load_pairs<P>(((t.x & kGopMirror) ? base2 : base1) + (long)(t.x & kGopIdx) * kSpecGroup, v);
 }
 
 // k_grid's accumulation of one tap (and its merge, when marked).
@@ -395,9 +428,8 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
     const unsigned tt = A.tiles[ti];
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
-    const long pair_stride = (long)A.n_ang * A.half;
-    const float2 *base1 = A.spec1 + pair * pair_stride;
-    const float2 *base2 = A.spec2 + pair * pair_stride;
+    const long group = spec_group_offset(A, pair);
+    const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
     const uint2 *e = O.gop + (unsigned long)O.tile_off[ti] + lane;
     const unsigned n = O.tile_n[ti] & kGopRows;
     float4 acc[1] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
@@ -411,7 +443,7 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
             if (k0 + i >= n) t[i] = make_uint2(0u, 0u);
         }
 #pragma unroll
-        for (int i = 0; i < kHeavyBatch; i++) gop_gather<1>(t[i], base1, base2, pair_stride, 1, v[i]);
+        for (int i = 0; i < kHeavyBatch; i++) gop_gather<1>(t[i], base1, base2, v[i]);
 #pragma unroll
         for (int i = 0; i < kHeavyBatch; i++) gop_consume<ACCUM, 1>(t[i], v[i], acc, part);
     }
@@ -461,9 +493,8 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
         acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         part[p] = make_float2(0.0f, 0.0f);
     }
-    const long pair_stride = (long)A.n_ang * A.half;
-    const float2 *base1 = A.spec1 + pair0 * pair_stride;
-    const float2 *base2 = A.spec2 + pair0 * pair_stride;
+    const long group = spec_group_offset(A, pair0);
+    const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
     const uint2 *e = O.gop + (unsigned long)O.tile_off[ti] + lane;
     const unsigned n = O.tile_n[ti] & kGopRows;
     if (n) {
@@ -471,16 +502,16 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
         auto row = [&](unsigned k) { return gop_load(e + 64ul * min(k, n - 1)); };
         uint2 t0 = row(0), t1 = row(1), t2 = row(2), t3 = row(3);
         float2 va[P], vb[P];
-        gop_gather<P>(t0, base1, base2, pair_stride, nvalid, va);
+        gop_gather<P>(t0, base1, base2, va);
         for (unsigned k = 0; k < n; k += 2) {
             // rows k (t0, samples in va) and k+1 (t1); t2, t3 hold rows k+2, k+3
             const uint2 c1 = k + 1 < n ? t1 : zero;
-            gop_gather<P>(c1, base1, base2, pair_stride, nvalid, vb);
+            gop_gather<P>(c1, base1, base2, vb);
             const uint2 c0 = t0;
             t0 = t2;
             t2 = row(k + 4);
             gop_consume<ACCUM, P>(c0, va, acc, part);
-            gop_gather<P>(k + 2 < n ? t0 : zero, base1, base2, pair_stride, nvalid, va);
+            gop_gather<P>(k + 2 < n ? t0 : zero, base1, base2, va);
             t1 = t3;
             t3 = row(k + 5);
             gop_consume<ACCUM, P>(c1, vb, acc, part);
@@ -536,7 +567,7 @@ GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) 
     A.sector_len = t.sector_len;
     A.n_ang = n_ang;
     A.half = (int)g.half;
-    A.lds_wtbl = (int)((ltbl + 1 + 1) & ~1L);  // keep the staging area 8-byte aligned
+    A.lds_wtbl = (int)((ltbl + 1 + 3) & ~3L);  // keep the staging area 16-byte aligned
     A.pair_base = 0;
     A.pair_end = 0;
     A.g.scale = g.scale;

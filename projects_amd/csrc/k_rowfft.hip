@@ -29,7 +29,7 @@ struct RowFftArgs {
     const float *sino;
     long slice_stride, row_stride;
     GrPre pre;
-    int n_slices, pair0, n_ang;
+    int n_slices, pair0, n_ang, pairs;
     int n_in, n_eff, pad_off, pad;
     int pdim, half, flag, radix;
     const float2 *filphase;
@@ -71,11 +71,17 @@ __device__ __forceinline__ float shifted(const unsigned short *raw, const float 
     return shift_interp([&](int q) { return normalised<PRE>(raw, dark, w1, w2, q); }, n, xoffset, k);
 }
 
+// Workgroup b takes angle (b / 8 / pairs) * 8 + b % 8 of pair (b / 8) % pairs:
+// the pairs of one angle run back to back on one XCD (workgroup b runs on XCD
+// b mod 8), where their 8-byte stores into the interleaved spectra fill whole
+// lines of that XCD's L2.
 template <int MAXB, int PRE>
 __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     extern __shared__ float2 row[];
-    const int n = blockIdx.x;
-    const int p = blockIdx.y;
+    const int r = blockIdx.x >> 3;
+    const int p = r % A.pairs;
+    const int n = (r / A.pairs) * 8 + (blockIdx.x & 7);
+    if (n >= A.n_ang) return;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     const bool has2 = s2 < A.n_slices;
@@ -113,8 +119,9 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     __syncthreads();
     gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr, A.radix);
 
-    float2 *o1 = A.spec1 + ((long)p * A.n_ang + n) * A.half;
-    float2 *o2 = A.spec2 + ((long)p * A.n_ang + n) * A.half;
+    const long rec = ((long)(p / kSpecGroup) * A.n_ang + n) * A.half * kSpecGroup + p % kSpecGroup;
+    float2 *o1 = A.spec1 + rec;
+    float2 *o2 = A.spec2 + rec;
     const float offset = A.roi_offset[n];
     for (int j = tid; j < A.half; j += nthr) {
         if (j == 0) {
@@ -122,6 +129,7 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
             o2[0] = make_float2(0.0f, 0.0f);
             continue;
         }
+        const long o = (long)j * kSpecGroup;
         float2 ct = A.filphase[j];
         if (A.flag) {  // grid.cpp:267-269
             const float arg = (float)j * offset;
@@ -129,9 +137,9 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
             ct = make_float2(ct.x * phr - ct.y * phi, ct.x * phi + ct.y * phr);
         }
         const float2 P = row[j], Q = row[A.pdim - j];
-        o1[j] = make_float2(ct.x * P.x - ct.y * P.y, ct.x * P.y + ct.y * P.x);
+        o1[o] = make_float2(ct.x * P.x - ct.y * P.y, ct.x * P.y + ct.y * P.x);
         const float ci = -ct.y;
-        o2[j] = make_float2(ct.x * Q.x - ci * Q.y, ct.x * Q.y + ci * Q.x);
+        o2[o] = make_float2(ct.x * Q.x - ci * Q.y, ct.x * Q.y + ci * Q.x);
     }
 }
 
@@ -149,6 +157,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     A.n_slices = n_slices;
     A.pair0 = pair0;
     A.n_ang = n_ang;
+    A.pairs = pairs;
     A.n_in = g.n_det_in;
     A.n_eff = g.n_det_eff;
     A.pad_off = g.n_det_in / 2;
@@ -164,7 +173,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     A.spec2 = spec2;
     const int nthr = gr_fft_threads(g.pdim);
     const long per = (g.pdim / 4 + nthr - 1) / nthr;
-    const dim3 grid(n_ang, pairs), block(nthr);
+    const dim3 grid((unsigned)((n_ang + 7) / 8 * 8 * pairs)), block(nthr);
     const size_t lds = sizeof(float2) * g.pdim;
 #define GR_LAUNCH_P(MB, PR)                                             \
     do {                                                                \
