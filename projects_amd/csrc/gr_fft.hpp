@@ -14,9 +14,21 @@
 //
 // tw[m] = e^{SIGN 2 pi i m / N}, m < N, rounded from double on the host.
 // MAXB bounds the radix-4 butterflies per thread: N/4 <= MAXB * nthreads.
+//
+// LDS layout: element i of the row lives at gr_lds_idx(i) = i XOR ((i >> 4)
+// mod 32), a permutation of each aligned block of 32 slots (the identity
+// below 32 points).  The first radix-16 pass writes 16 consecutive outputs
+// per thread, so in natural order the lanes of a store sit 128 bytes apart,
+// all in the same banks; the swizzle spreads every pass's loads and stores
+// over the banks (checked against 32- and 64-bank models for 2048 .. 8192
+// points) without extra LDS.  Callers that fill the row or read the result
+// use the same mapping.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+__host__ __device__ __forceinline__ int gr_lds_idx(int i) { return i ^ ((i >> 4) & 31); }
+__host__ __device__ __forceinline__ long gr_lds_len(long N) { return N; }
 
 __device__ __forceinline__ float2 gr_cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -74,7 +86,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             if (j < nb16) {
                 const int k = j & (Ns - 1);
 #pragma unroll
-                for (int r = 0; r < 16; r++) v[c][r] = buf[j + r * nb16];
+                for (int r = 0; r < 16; r++) v[c][r] = buf[gr_lds_idx(j + r * nb16)];
                 if (Ns > 1) {
                     const int m = k * stride;
 #pragma unroll
@@ -91,7 +103,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
                 const int k = j & (Ns - 1);
                 const int base = ((j - k) << 4) + k;
 #pragma unroll
-                for (int q = 0; q < 16; q++) buf[base + q * Ns] = v[c][4 * (q & 3) + (q >> 2)];
+                for (int q = 0; q < 16; q++) buf[gr_lds_idx(base + q * Ns)] = v[c][4 * (q & 3) + (q >> 2)];
             }
         }
         __syncthreads();
@@ -107,7 +119,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             if (j < nb4) {
                 const int k = j & (Ns - 1);
 #pragma unroll
-                for (int r = 0; r < 4; r++) v[c][r] = buf[j + r * nb4];
+                for (int r = 0; r < 4; r++) v[c][r] = buf[gr_lds_idx(j + r * nb4)];
                 if (Ns > 1) {
                     const int m = k * stride;
                     v[c][1] = gr_cmul(v[c][1], tw[m]);
@@ -125,7 +137,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
                 const int k = j & (Ns - 1);
                 const int base = ((j - k) << 2) + k;
 #pragma unroll
-                for (int r = 0; r < 4; r++) buf[base + r * Ns] = v[c][r];
+                for (int r = 0; r < 4; r++) buf[gr_lds_idx(base + r * Ns)] = v[c][r];
             }
         }
         __syncthreads();
@@ -139,8 +151,8 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             const int j = tid + c * nthr;
             if (j < nb2) {
                 const int k = j & (Ns - 1);
-                const float2 x0 = buf[j];
-                const float2 x1 = gr_cmul(buf[j + nb2], tw[k]);
+                const float2 x0 = buf[gr_lds_idx(j)];
+                const float2 x1 = gr_cmul(buf[gr_lds_idx(j + nb2)], tw[k]);
                 a[c] = make_float2(x0.x + x1.x, x0.y + x1.y);
                 b[c] = make_float2(x0.x - x1.x, x0.y - x1.y);
             }
@@ -152,8 +164,8 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             if (j < nb2) {
                 const int k = j & (Ns - 1);
                 const int base = ((j - k) << 1) + k;
-                buf[base] = a[c];
-                buf[base + Ns] = b[c];
+                buf[gr_lds_idx(base)] = a[c];
+                buf[gr_lds_idx(base + Ns)] = b[c];
             }
         }
         __syncthreads();

@@ -47,7 +47,7 @@
 namespace {
 
 constexpr int kTile = 8;        // cells per tile side; one tile per wavefront
-constexpr int kMergeAngles = 4;  // sample lines per compensated partial
+constexpr int kMergeAngles = 16;  // sample lines per compensated partial
 
 struct GridArgs {
     const float2 *spec1, *spec2;

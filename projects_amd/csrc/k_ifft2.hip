@@ -36,11 +36,11 @@ __global__ void __launch_bounds__(512) k_ifft_rows(RowsArgs A) {
     const int b = blockIdx.x, p = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const float2 *src = A.ht + ((long)p * A.M + b) * A.M;
-    for (int i = tid; i < A.M; i += nthr) row[i] = src[i];
+    for (int i = tid; i < A.M; i += nthr) row[gr_lds_idx(i)] = src[i];
     __syncthreads();
     gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr, A.radix);
     float2 *dst = A.z + ((long)p * A.M + b) * A.out_n;
-    for (long jj = tid; jj < A.out_n; jj += nthr) dst[jj] = row[corner_index(A.out_off + jj, A.M, A.M02)];
+    for (long jj = tid; jj < A.out_n; jj += nthr) dst[jj] = row[gr_lds_idx((int)corner_index(A.out_off + jj, A.M, A.M02))];
 }
 
 struct ColsArgs {
@@ -64,7 +64,7 @@ __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
 
 template <int MAXB>
 __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
-    extern __shared__ float2 cols[];  // cw columns of M
+    extern __shared__ float2 cols[];  // cw columns of gr_lds_len(M) slots
     const int p = blockIdx.y;
     const long c0 = column_group(blockIdx.x, gridDim.x) * A.cw;
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -72,10 +72,10 @@ __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
     const float2 *zp = A.z + (long)p * M * A.out_n;
     for (long idx = tid; idx < (long)cw * M; idx += nthr) {
         const long b = idx / cw, c = idx - b * cw;
-        cols[c * M + b] = (c0 + c < A.out_n) ? zp[b * A.out_n + c0 + c] : make_float2(0.0f, 0.0f);
+        cols[c * gr_lds_len(M) + gr_lds_idx((int)b)] = (c0 + c < A.out_n) ? zp[b * A.out_n + c0 + c] : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
-    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + (long)c * M, M, A.tw, tid, nthr, A.radix);
+    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
 
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     float *o1 = A.out + s1 * A.out_slice_stride;
@@ -84,11 +84,11 @@ __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
         const long jj = c0 + c;
         if (jj >= A.out_n) break;
         const float cu = A.winv[A.out_off + jj];
-        const float2 *col = cols + (long)c * M;
+        const float2 *col = cols + c * gr_lds_len(M);
         for (long kk = tid; kk < A.out_n; kk += nthr) {
             const long k = A.out_off + kk;
             const float corrn = cu * A.winv[k];  // grid.cpp:401
-            const float2 v = col[corner_index(k, M, A.M02)];
+            const float2 v = col[gr_lds_idx((int)corner_index(k, M, A.M02))];
             o1[jj * A.out_row_stride + kk] = corrn * v.x;
             if (o2) o2[jj * A.out_row_stride + kk] = corrn * v.y;
         }
@@ -104,7 +104,7 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
     RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n};
     const int nthr = gr_fft_threads(g.M);
     const dim3 grid((unsigned)g.M, pairs), block(nthr);
-    const size_t lds = sizeof(float2) * g.M;
+    const size_t lds = sizeof(float2) * gr_lds_len(g.M);
     switch (maxb_of(g.M, nthr)) {
     case 0:
     case 1: { hipError_t e_ = gr_allow_lds(k_ifft_rows<1>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_rows<1>, grid, block, lds, s, A); } break;
@@ -150,7 +150,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_n = g.out_n;
     const int nthr = gr_fft_threads(g.M);
     const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(nthr);
-    const size_t lds = sizeof(float2) * A.cw * g.M;
+    const size_t lds = sizeof(float2) * A.cw * gr_lds_len(g.M);
     switch (maxb_of(g.M, nthr)) {
     case 0:
     case 1: { hipError_t e_ = gr_allow_lds(k_ifft_cols<1>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<1>, grid, block, lds, s, A); } break;

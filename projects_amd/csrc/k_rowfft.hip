@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
                                       P.white1 + f2, P.white2 + f2, A.n_in, P.xoffset, q);
             }
         }
-        row[j] = make_float2(re, im);
+        row[gr_lds_idx(j)] = make_float2(re, im);
     }
     __syncthreads();
     gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr, A.radix);
@@ -136,7 +136,7 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
             const float phr = cosf(arg), phi = -sinf(arg);
             ct = make_float2(ct.x * phr - ct.y * phi, ct.x * phi + ct.y * phr);
         }
-        const float2 P = row[j], Q = row[A.pdim - j];
+        const float2 P = row[gr_lds_idx(j)], Q = row[gr_lds_idx(A.pdim - j)];
         o1[o] = make_float2(ct.x * P.x - ct.y * P.y, ct.x * P.y + ct.y * P.x);
         const float ci = -ct.y;
         o2[o] = make_float2(ct.x * Q.x - ci * Q.y, ct.x * Q.y + ci * Q.x);
@@ -174,7 +174,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     const int nthr = gr_fft_threads(g.pdim);
     const long per = (g.pdim / 4 + nthr - 1) / nthr;
     const dim3 grid((unsigned)((n_ang + 7) / 8 * 8 * pairs)), block(nthr);
-    const size_t lds = sizeof(float2) * g.pdim;
+    const size_t lds = sizeof(float2) * gr_lds_len(g.pdim);
 #define GR_LAUNCH_P(MB, PR)                                             \
     do {                                                                \
         hipError_t e_ = gr_allow_lds(k_rowfft<MB, PR>, lds);            \
