@@ -527,6 +527,10 @@ def main():
         b = kernel_bytes(kname, nang, ndet, half, M, out_n, pairs_per_launch)
         per_kernel[kname] = dict(total_ms=round(tim[ms_f], 3), launches=tim[n_f], avg_ms=round(avg_ms, 4),
                                  alg_bytes=int(b), gbs=round(b / (avg_ms * 1e-3) / 1e9, 1))
+        if kname == "grid" and grid_op_used == "on":
+            # the precomputed gather operator, streamed once per launch on top
+            # of the algorithmic bytes
+            per_kernel[kname]["operator_bytes"] = int(inf["grid_op_bytes"])
     dom = max(per_kernel, key=lambda k: per_kernel[k]["total_ms"])
     d = per_kernel[dom]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
