@@ -66,6 +66,15 @@ __host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float q
 }
 
 
+// Whether the 8 x 8 tile (tx, ty) can hold a nonzero cell: its centre within
+// rmax (the largest sample radius plus the reach of a tap) of the grid centre.
+// The gridding kernels write only such tiles and the row transform reads only
+// them (the others are zero); both evaluate this same expression.
+__host__ __device__ __forceinline__ bool gr_tile_active(int tx, int ty, float M2, float rmax) {
+    const float cu = (float)(tx * 8) + 3.5f - M2, cv = (float)(ty * 8) + 3.5f - M2;
+    return sqrtf(cu * cu + cv * cv) <= rmax;
+}
+
 // Angle window of an 8 x 8 tile with origin (tx*8, ty*8): the ranges
 // [r0, r1) and [r2, r3) of sorted angle keys (akey, ascending in [0, pi)) whose
 // sample line passes within dmax of the tile centre.  A line of direction

@@ -89,6 +89,10 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
 // K2: tile-local gridding into the transposed grid Ht[pairs][M][M] (Ht[b][a] = H[a][b]).
 // The plan lists the kernel's tiles (gr_grid_tile_size() cells per side).
 int gr_grid_tile_size();
+// Radius (cells, from the grid centre) beyond which tiles hold no nonzero
+// cell: gridding leaves them unwritten, the row transform reads them as zero
+// (gr_tile_active).
+float gr_grid_rmax(const GrGeom &g);
 // Sectors of the tile list: one per XCD of the MI355X (workgroup b runs on
 // XCD b mod 8), and the padding value of a sector's unused slots.
 constexpr int kGridSectors = 8;

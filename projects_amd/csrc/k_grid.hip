@@ -169,8 +169,8 @@ template <class ChunkFn, class LineFn>
 __device__ __forceinline__ void walk_tile(const GridArgs &A, int tx, int ty, bool valid, float du, float dv,
                                           ChunkFn &&on_chunk, LineFn &&on_line) {
     const float M2 = A.g.fM2;
+    if (!gr_tile_active(tx, ty, M2, A.rmax)) return;
     const float cu = (float)(tx * kTile) + 3.5f - M2, cv = (float)(ty * kTile) + 3.5f - M2;
-    if (!(sqrtf(cu * cu + cv * cv) <= A.rmax)) return;
     const float reach = 3.5f + A.L2m;  // tile half-width plus box half-width
     int r0, r1, r2, r3;
     gr_angle_window(tx, ty, M2, A.dmax, A.akey, A.n_ang, r0, r1, r2, r3);
@@ -201,10 +201,13 @@ __device__ __forceinline__ void walk_tile(const GridArgs &A, int tx, int ty, boo
     }
 }
 
+// Write the wave's 64 cells for the P pairs -- only for tiles that can hold
+// a nonzero cell: the row transform reads the other tiles as zeros.
 template <int ACCUM, int P>
 __device__ __forceinline__ void store_cells(const GridArgs &A, int pair0, int nvalid, int a, int b,
                                             const float4 (&acc)[P]) {
     const int M = A.g.M;
+    if (!gr_tile_active(a / kTile, b / kTile, A.g.fM2, A.rmax)) return;
     if (a < M && b < M) {
 #pragma unroll
         for (int p = 0; p < P; p++) {
@@ -554,6 +557,17 @@ hipError_t launch_all(GridArgs A, const GopArgs *O, int pairs, size_t lds_base, 
     return launch<ACCUM, 1>(A, O, 1, lds_base, wpb, s);
 }
 
+}  // namespace
+
+float gr_grid_rmax(const GrGeom &g) {
+    const float L2m = g.L2 + 0.01f;
+    const float dmax = 3.5f * 1.41421356f + L2m * 1.41421356f + 0.05f;
+    // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
+    return g.scale * (float)(g.half - 1) * 1.0001f + dmax + 0.5f;
+}
+
+namespace {
+
 GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) {
     GridArgs A;
     A.spec1 = A.spec2 = nullptr;
@@ -579,8 +593,7 @@ GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) 
     A.g.ltbl = (int)ltbl;
     A.L2m = g.L2 + 0.01f;
     A.dmax = 3.5f * 1.41421356f + A.L2m * 1.41421356f + 0.05f;
-    // largest sample radius (|(COSE,SINE)| <= 1 + 1e-4 for the float recurrence)
-    A.rmax = g.scale * (float)(g.half - 1) * 1.0001f + A.dmax + 0.5f;
+    A.rmax = gr_grid_rmax(g);
     return A;
 }
 

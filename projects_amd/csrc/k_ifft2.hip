@@ -12,6 +12,7 @@
 // n columns down b, keeps the n outputs q = iv(k) and writes the two images
 // row by row with the deapodization product fused in.
 #include "gr_fft.hpp"
+#include "gr_gather.hpp"
 #include "gr_internal.h"
 
 #pragma clang fp contract(off)
@@ -28,6 +29,7 @@ struct RowsArgs {
     const float2 *tw;
     int M, radix;
     long M02, out_off, out_n;
+    float rmax;  // tiles beyond it were not written by the gridding step: zeros
 };
 
 template <int MAXB>
@@ -36,7 +38,9 @@ __global__ void __launch_bounds__(512) k_ifft_rows(RowsArgs A) {
     const int b = blockIdx.x, p = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
     const float2 *src = A.ht + ((long)p * A.M + b) * A.M;
-    for (int i = tid; i < A.M; i += nthr) row[gr_lds_idx(i)] = src[i];
+    const float M2 = (float)(A.M >> 1);
+    for (int i = tid; i < A.M; i += nthr)
+        row[gr_lds_idx(i)] = gr_tile_active(i >> 3, b >> 3, M2, A.rmax) ? src[i] : make_float2(0.0f, 0.0f);
     __syncthreads();
     gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr, A.radix);
     float2 *dst = A.z + ((long)p * A.M + b) * A.out_n;
@@ -101,7 +105,7 @@ inline int maxb_of(long N, int nthr) { return (int)((N / 4 + nthr - 1) / nthr); 
 
 hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
                                int pairs, hipStream_t s) {
-    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n};
+    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n, gr_grid_rmax(g)};
     const int nthr = gr_fft_threads(g.M);
     const dim3 grid((unsigned)g.M, pairs), block(nthr);
     const size_t lds = sizeof(float2) * gr_lds_len(g.M);
