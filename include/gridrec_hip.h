@@ -112,6 +112,12 @@ typedef struct gr_params {
     int device;            /* HIP device ordinal */
     int max_pairs;         /* slice pairs per launch (workspace sizing); 0 = automatic */
     int grid_op;           /* GR_GRIDOP_* */
+    int interp;            /* 1: the convolvent of the reference's compile-time INTERP variant
+                              (grid.cpp:18-24, 487-489): linear table interpolation with the
+                              backward difference wtbl[i] - wtbl[i-1], dwtbl[0] = 0 (the
+                              reference never sets it; its macro does not compile as written);
+                              0 (default): nearest neighbour, as the reference ships.
+                              GR_TABLES_REFERENCE only */
     /* optional overrides, as recon_init honours grid_struct.filter and
        grid_struct.pswf (grid.cpp:78-81): a filter function evaluated on the
        host for the filter table (only with GR_TABLES_REFERENCE), and a PSWF

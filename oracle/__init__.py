@@ -44,6 +44,7 @@ class Geometry:
     Y0: float = 0.0
     ltbl: int = 512
     angles: Optional[Sequence[float]] = field(default=None, repr=False)
+    interp: bool = False  # the INTERP convolvent (gridrec_oracle.h: or_params.interp)
 
     def padded(self) -> "Geometry":
         """Geometry seen by gridrec after the half-padding caller convention
@@ -59,7 +60,7 @@ class _OrParams(C.Structure):
         ("n_ang", C.c_int), ("n_det", C.c_int), ("geom", C.c_int), ("filter_id", C.c_int),
         ("center", C.c_float), ("C", C.c_float), ("sampl", C.c_float), ("MaxPixSiz", C.c_float),
         ("R", C.c_float), ("X0", C.c_float), ("Y0", C.c_float), ("ltbl", C.c_long),
-        ("angles", C.POINTER(C.c_float)),
+        ("angles", C.POINTER(C.c_float)), ("interp", C.c_int),
     ]
 
 
@@ -152,7 +153,7 @@ def _params(g: Geometry):
         keep = np.ascontiguousarray(np.asarray(g.angles, dtype=np.float32))
         ang = _fptr(keep)
     p = _OrParams(g.n_ang, g.n_det, g.geom, FILTERS[g.filter], g.center, g.C, g.sampl,
-                  g.MaxPixSiz, g.R, g.X0, g.Y0, g.ltbl, ang)
+                  g.MaxPixSiz, g.R, g.X0, g.Y0, g.ltbl, ang, int(bool(g.interp)))
     return p, keep
 
 

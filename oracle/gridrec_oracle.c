@@ -198,19 +198,38 @@ static double legendre_d(int n, const float *coefs, double x) {
  * 0.5 double. */
 static inline long cidx(float X) { return (long)((double)X + 0.5); }
 
-/* pswf_su (grid.cpp:463-512).  sampl is the adjusted value (grid.cpp:117). */
+/* Cnvlvnt of the INTERP variant (grid.cpp:19) read with its argument
+ * parenthesised -- as written the macro does not compile: wtbl[(int)X] +
+ * (X - (int)X) * dwtbl[(int)X], all float, with dwtbl[i] = wtbl[i] - wtbl[i-1]
+ * (grid.cpp:488); dwtbl[0], never set by the reference, is 0 here. */
+static inline float cnv_interp(const float *wtbl, const float *dwtbl, float X) {
+    const int i = (int)X;
+    return wtbl[i] + (X - (float)i) * dwtbl[i];
+}
+
+/* The table value of the convolvent at X (grid.cpp:18-24): nearest neighbour,
+ * or linear interpolation when dwtbl is given. */
+static inline float cnv(const float *wtbl, const float *dwtbl, float X) {
+    return dwtbl ? cnv_interp(wtbl, dwtbl, X) : wtbl[cidx(X)];
+}
+
+/* pswf_su (grid.cpp:463-512).  sampl is the adjusted value (grid.cpp:117).
+ * dwtbl (ltbl+1 floats) is filled and used when not NULL (INTERP). */
 static void pswf_f(const or_pswf *ps, long ltbl, long linv, float sampl,
-                   float *wtbl, float *winv) {
+                   float *wtbl, float *winv, float *dwtbl) {
     float polyz = legendre_f(ps->nt, ps->coefs, 0.0f);
     wtbl[0] = 1.0f;
-    for (long i = 1; i <= ltbl; i++)
+    if (dwtbl) dwtbl[0] = 0.0f;
+    for (long i = 1; i <= ltbl; i++) {
         wtbl[i] = legendre_f(ps->nt, ps->coefs, (float)i / (float)ltbl) / polyz;
+        if (dwtbl) dwtbl[i] = wtbl[i] - wtbl[i - 1];                   /* grid.cpp:488 */
+    }
     float fac = (float)((double)(float)ltbl / ((double)linv + 0.5)); /* grid.cpp:492 */
     float norm = (float)(sqrt(OR_PI / 2 / (double)ps->C / (double)ps->lmbda) / (double)sampl);
-    winv[linv] = norm / wtbl[cidx(0.0f)];
+    winv[linv] = norm / cnv(wtbl, dwtbl, 0.0f);
     for (long i = 1; i <= linv; i++) {
         norm = -norm;
-        float v = norm / wtbl[cidx((float)i * fac)];
+        float v = norm / cnv(wtbl, dwtbl, (float)i * fac);
         winv[linv + i] = v;
         winv[linv - i] = v;
     }
@@ -267,10 +286,15 @@ int or_tables(const or_params *p, float *sine, float *cose, float *filphase,
     float *S = malloc(sizeof(float) * p->n_ang), *C = malloc(sizeof(float) * p->n_ang);
     float *W = malloc(sizeof(float) * (p->ltbl + 1)), *WI = malloc(sizeof(float) * s.M0);
     float *F = malloc(sizeof(float) * s.pdim);
-    if (!S || !C || !W || !WI || !F) { free(S); free(C); free(W); free(WI); free(F); return -1; }
+    float *DW = p->interp ? malloc(sizeof(float) * (p->ltbl + 1)) : NULL;
+    if (!S || !C || !W || !WI || !F || (p->interp && !DW)) {
+        free(S); free(C); free(W); free(WI); free(F); free(DW);
+        return -1;
+    }
     trig_f(p, S, C);
     filphase_f(p, s.pdim, F);
-    pswf_f(ps, p->ltbl, s.M02, s.sampl, W, WI);
+    pswf_f(ps, p->ltbl, s.M02, s.sampl, W, WI, DW);
+    free(DW);
     if (sine) memcpy(sine, S, sizeof(float) * p->n_ang);
     if (cose) memcpy(cose, C, sizeof(float) * p->n_ang);
     if (filphase) memcpy(filphase, F, sizeof(float) * s.pdim);
@@ -410,14 +434,16 @@ int or_recon_f32(const or_params *p, const float *g1, const float *g2, float *s1
     float *SINE = malloc(sizeof(float) * n_ang), *COSE = malloc(sizeof(float) * n_ang);
     float *filphase = malloc(sizeof(float) * pdim);
     float *wtbl = malloc(sizeof(float) * (p->ltbl + 1)), *winv = malloc(sizeof(float) * M0);
+    float *dwtbl = p->interp ? malloc(sizeof(float) * (p->ltbl + 1)) : NULL;
     cf *cproj = malloc(sizeof(cf) * pdim);
     cf *H = calloc((size_t)M * (size_t)M, sizeof(cf));            /* grid.cpp:165-170 */
     double *twr = twiddles(pdim), *twm = twiddles(M);
     float *twrf = twr ? twiddles_f(twr, pdim) : NULL, *twmf = twm ? twiddles_f(twm, M) : NULL;
     if (!SINE || !COSE || !filphase || !wtbl || !winv || !cproj || !H || !twrf || !twmf) goto out;
+    if (p->interp && !dwtbl) goto out;
     trig_f(p, SINE, COSE);
     filphase_f(p, pdim, filphase);
-    pswf_f(ps, p->ltbl, M02, s.sampl, wtbl, winv);
+    pswf_f(ps, p->ltbl, M02, s.sampl, wtbl, winv, dwtbl);
 
     const float L2 = s.L2, tblspcg = s.tblspcg, scale = s.scale, fM2 = (float)M2;
     for (int n = 0; n < n_ang; n++) {                                /* grid.cpp:234 */
@@ -458,11 +484,11 @@ int or_recon_f32(const or_params *p, const float *g1, const float *g2, float *s1
             long k = 0;
             for (long iv = ivl; iv <= ivh && k < 64; iv++, k++) {
                 float d = V - (float)iv;
-                work[k] = wtbl[cidx((d > 0 ? d : -d) * tblspcg)];       /* grid.cpp:290 */
+                work[k] = cnv(wtbl, dwtbl, (d > 0 ? d : -d) * tblspcg);  /* grid.cpp:290 */
             }
             for (long iu = iul; iu <= iuh; iu++) {
                 float d = U - (float)iu;
-                float rt = wtbl[cidx((d > 0 ? d : -d) * tblspcg)];      /* grid.cpp:293 */
+                float rt = cnv(wtbl, dwtbl, (d > 0 ? d : -d) * tblspcg); /* grid.cpp:293 */
                 cf *row = H + iu * M, *mrow = H + (M - iu) * M;
                 k = 0;
                 for (long iv = ivl; iv <= ivh && k < 64; iv++, k++) {
@@ -488,7 +514,7 @@ int or_recon_f32(const or_params *p, const float *g1, const float *g2, float *s1
     }
     rc = 0;
 out:
-    free(SINE); free(COSE); free(filphase); free(wtbl); free(winv); free(cproj); free(H);
+    free(SINE); free(COSE); free(filphase); free(wtbl); free(winv); free(dwtbl); free(cproj); free(H);
     free(twr); free(twm); free(twrf); free(twmf);
     return rc;
 }
@@ -617,9 +643,11 @@ out:
 }
 
 int or_recon_f64(const or_params *p, const float *g1, const float *g2, double *s1, double *s2) {
+    if (p->interp) return -1;
     return recon_d(p, g1, g2, s1, s2, 0);
 }
 int or_recon_exact64(const or_params *p, const float *g1, const float *g2, double *s1, double *s2) {
+    if (p->interp) return -1;
     return recon_d(p, g1, g2, s1, s2, 1);
 }
 
@@ -651,6 +679,7 @@ static void fft_mx(cd *x, long n, int sign, const double *tw, int r32) {
 
 int or_recon_mixed(const or_params *p, const float *g1, const float *g2, double *s1, double *s2,
                    int mask) {
+    if (p->interp) return -1;
     or_sizes s;
     if (or_sizes_get(p, &s)) return -1;
     const or_pswf *ps = find_pswf(p->C);
@@ -672,7 +701,7 @@ int or_recon_mixed(const or_params *p, const float *g1, const float *g2, double 
     if (td) { filphase_d(p, pdim, fp); pswf_d(ps, p->ltbl, M02, s.sampl, wt, wi); }
     else {
         filphase_f(p, pdim, fpf);
-        pswf_f(ps, p->ltbl, M02, s.sampl, wtf, wif);
+        pswf_f(ps, p->ltbl, M02, s.sampl, wtf, wif, NULL);
         for (long j = 0; j < pdim; j++) fp[j] = fpf[j];
         for (long j = 0; j <= p->ltbl; j++) wt[j] = wtf[j];
         for (long j = 0; j < M0; j++) wi[j] = wif[j];

@@ -37,6 +37,16 @@ typedef struct or_params {
     float X0, Y0;         /* grid.h:52-53 */
     long ltbl;            /* grid.h:56 */
     const float *angles;  /* degrees, used when geom == 0 (grid.h:32) */
+    int interp;           /* 1: the convolvent of the reference's compile-time INTERP variant
+                             (grid.cpp:18-24, 487-489): wtbl[i] + (X - i) * dwtbl[i], i = (int)X,
+                             with the backward difference dwtbl[i] = wtbl[i] - wtbl[i-1] and
+                             dwtbl[0] (never set there) taken as 0.  The reference's macro does
+                             not compile as written (its argument is not parenthesised, so
+                             Cnvlvnt(abs(V-iv)*tblspcg) subscripts wtbl with a float); this is
+                             its evident reading.  PARITY UNPINNED against the reference (no
+                             build of it exists); pinned by the convolvent's second-order
+                             accuracy against the exact PSWF.  or_tables, or_recon_f32 and
+                             or_recon_batch_f32 only (the FP64 variants return -1) */
 } or_params;
 
 typedef struct or_sizes {

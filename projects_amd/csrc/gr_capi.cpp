@@ -222,7 +222,8 @@ std::vector<unsigned> sector_tiles(int nt, int ts, long M) {
 void free_plan(gr_plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
-                    pl->d_aord, pl->d_ang, pl->d_in, pl->d_out, pl->d_gop, pl->d_gop_n, pl->d_gop_off};
+                    pl->d_aord, pl->d_ang, pl->d_in, pl->d_out, pl->d_gop, pl->d_gop_n, pl->d_gop_off,
+                    pl->d_dwtbl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -404,13 +405,16 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
     // reference tables (host, bit-identical to recon_init)
     std::vector<float> sine(n_ang), cose(n_ang), roi(n_ang), filphase(2 * g.half), wtbl(params->ltbl + 1),
         winv(g.M0);
-    gr_host_tables(&pl->prm, g, sine.data(), cose.data(), filphase.data(), wtbl.data(), winv.data(), roi.data());
+    std::vector<float> dwtbl(params->interp ? params->ltbl + 1 : 0);
+    gr_host_tables(&pl->prm, g, sine.data(), cose.data(), filphase.data(), wtbl.data(), winv.data(), roi.data(),
+                   params->interp ? dwtbl.data() : nullptr);
     CK(dev_upload(pl, &pl->d_sine, sine.data(), n_ang));
     CK(dev_upload(pl, &pl->d_cose, cose.data(), n_ang));
     CK(dev_upload(pl, &pl->d_roi, roi.data(), n_ang));
     CK(dev_upload(pl, &pl->d_filphase, (const float2 *)filphase.data(), g.half));
     CK(dev_upload(pl, &pl->d_wtbl, wtbl.data(), wtbl.size()));
     CK(dev_upload(pl, &pl->d_winv, winv.data(), winv.size()));
+    if (params->interp) CK(dev_upload(pl, &pl->d_dwtbl, dwtbl.data(), dwtbl.size()));
     if (params->tables == GR_TABLES_FP64) {
         hipStream_t ts;
         CK(plan_stream(pl, &ts));

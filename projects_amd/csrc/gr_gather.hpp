@@ -66,6 +66,30 @@ __host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float q
 }
 
 
+// Convolvent of the reference's compile-time INTERP variant (grid.cpp:18-24)
+// with its argument parenthesised (as written the macro does not compile:
+// Cnvlvnt(abs(V-iv)*tblspcg) would subscript wtbl with a float):
+// wtbl[i] + (X - i) * dwtbl[i], i = (int)X for X >= 0, dwtbl[i] = wtbl[i] -
+// wtbl[i-1] (grid.cpp:488) and dwtbl[0] = 0 (the reference never sets it).
+__host__ __device__ __forceinline__ float gr_cnv_interp(float X, const float *wtbl, const float *dwtbl, int ltbl) {
+    int i = (int)X;
+    i = i > ltbl ? ltbl : i;
+    return wtbl[i] + (X - (float)i) * dwtbl[i];
+}
+
+// gr_tap with the INTERP convolvent.
+__host__ __device__ __forceinline__ bool gr_tap_interp(int j, float C, float S, float qa, float qb,
+                                                       const GrGatherGeom &g, const float *wtbl,
+                                                       const float *dwtbl, float &w) {
+    float U, V;
+    gr_sample_uv(j, C, S, g, U, V);
+    if (!(U - g.L2 <= qa && qa <= U + g.L2 && V - g.L2 <= qb && qb <= V + g.L2)) return false;
+    const float cu = gr_cnv_interp(fabsf(U - qa) * g.tblspcg, wtbl, dwtbl, g.ltbl);
+    const float cv = gr_cnv_interp(fabsf(V - qb) * g.tblspcg, wtbl, dwtbl, g.ltbl);
+    w = cu * cv;
+    return true;
+}
+
 // Whether the 8 x 8 tile (tx, ty) can hold a nonzero cell: its centre within
 // rmax (the largest sample radius plus the reach of a tap) of the grid centre.
 // The gridding kernels write only such tiles and the row transform reads only

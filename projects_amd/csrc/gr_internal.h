@@ -31,9 +31,10 @@ int gr_geom_from_params(const gr_params *p, GrGeom *g, const char **why);
 int gr_find_pswf(float C, gr_pswf *out);
 int gr_filter_id_of(const char *name);
 // sine/cose: n_ang; filphase: 2*half floats; wtbl: ltbl+1; winv: M0;
-// roi_offset: n_ang (phase offset per angle, grid.cpp:237; zeros without ROI).
+// roi_offset: n_ang (phase offset per angle, grid.cpp:237; zeros without ROI);
+// dwtbl: ltbl+1 backward differences of wtbl (filled with p->interp; may be null).
 void gr_host_tables(const gr_params *p, const GrGeom &g, float *sine, float *cose,
-                    float *filphase, float *wtbl, float *winv, float *roi_offset);
+                    float *filphase, float *wtbl, float *winv, float *roi_offset, float *dwtbl);
 
 // Device tables and workspace of a plan.
 struct GrDevTables {
@@ -41,6 +42,7 @@ struct GrDevTables {
     const float *roi_offset;       // n_ang
     const float2 *filphase;        // half
     const float *wtbl;             // ltbl+1
+    const float *dwtbl = nullptr;  // ltbl+1 backward differences: the INTERP convolvent (null: nearest)
     const float *winv;             // M0
     const float2 *tw_row;          // pdim: e^{+2 pi i m/pdim}
     const float2 *tw_grid;         // M:    e^{-2 pi i m/M}

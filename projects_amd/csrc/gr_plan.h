@@ -13,6 +13,7 @@ struct gr_plan {
     hipStream_t stream = nullptr;
     // device tables
     float *d_cose = nullptr, *d_sine = nullptr, *d_roi = nullptr, *d_wtbl = nullptr, *d_winv = nullptr;
+    float *d_dwtbl = nullptr;  // INTERP only
     float2 *d_filphase = nullptr, *d_tw_row = nullptr, *d_tw_grid = nullptr;
     float *d_akey = nullptr;
     int *d_aord = nullptr;
@@ -57,6 +58,7 @@ struct gr_plan {
         t.roi_offset = d_roi;
         t.filphase = d_filphase;
         t.wtbl = d_wtbl;
+        t.dwtbl = d_dwtbl;
         t.winv = d_winv;
         t.tw_row = d_tw_row;
         t.tw_grid = d_tw_grid;

@@ -9,6 +9,10 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <vector>
+
+#include "gr_gather.hpp"
 #include "gr_internal.h"
 
 namespace {
@@ -115,6 +119,8 @@ int gr_geom_from_params(const gr_params *p, GrGeom *g, const char **why) {
         return GR_ENOPSWF;
     }
     if (p->filter_fn && p->tables == GR_TABLES_FP64) { *why = "a filter function needs GR_TABLES_REFERENCE"; return GR_EINVAL; }
+    if (p->interp != 0 && p->interp != 1) { *why = "interp must be 0 or 1"; return GR_EINVAL; }
+    if (p->interp && p->tables == GR_TABLES_FP64) { *why = "interp needs GR_TABLES_REFERENCE"; return GR_EINVAL; }
     int fid = gr_filter_id_of(p->filter);
     g->filter_id = fid < 0 ? 0 : fid;  // unknown name: default shlo (filters.cpp:43-47)
 
@@ -171,7 +177,7 @@ int gr_geom_from_params(const gr_params *p, GrGeom *g, const char **why) {
 }
 
 void gr_host_tables(const gr_params *p, const GrGeom &g, float *sine, float *cose, float *filphase,
-                    float *wtbl, float *winv, float *roi_offset) {
+                    float *wtbl, float *winv, float *roi_offset, float *dwtbl) {
     const int n_ang = p->n_ang;
     // trig_su (grid.cpp:515-563)
     if (p->geom == 0) {
@@ -219,12 +225,23 @@ void gr_host_tables(const gr_params *p, const GrGeom &g, float *sine, float *cos
     const float polyz = legendre_sum(ps.nt, ps.coefs, 0.0f);
     wtbl[0] = 1.0f;
     for (long i = 1; i <= ltbl; i++) wtbl[i] = legendre_sum(ps.nt, ps.coefs, (float)i / (float)ltbl) / polyz;
+    // INTERP (grid.cpp:487-489): backward differences, dwtbl[0] = 0
+    std::vector<float> dw(p->interp ? ltbl + 1 : 0);
+    if (p->interp) {
+        dw[0] = 0.0f;
+        for (long i = 1; i <= ltbl; i++) dw[i] = wtbl[i] - wtbl[i - 1];
+        if (dwtbl) std::copy(dw.begin(), dw.end(), dwtbl);
+    }
+    auto cnv = [&](float X) {  // Cnvlvnt (grid.cpp:18-24)
+        if (!p->interp) return wtbl[nn_index(X)];
+        return gr_cnv_interp(X, wtbl, dw.data(), (int)ltbl);
+    };
     const float fac = (float)((double)(float)ltbl / ((double)linv + 0.5));
     float wn = (float)(sqrt(kRefPi / 2 / (double)ps.C / (double)ps.lmbda) / (double)g.sampl);
-    winv[linv] = wn / wtbl[nn_index(0.0f)];
+    winv[linv] = wn / cnv(0.0f);
     for (long i = 1; i <= linv; i++) {
         wn = -wn;
-        const float v = wn / wtbl[nn_index((float)i * fac)];
+        const float v = wn / cnv((float)i * fac);
         winv[linv + i] = v;
         winv[linv - i] = v;
     }

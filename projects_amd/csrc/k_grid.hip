@@ -57,6 +57,7 @@ struct GridArgs {
     const float *akey;   // sorted keys
     const unsigned *tiles;
     const float *wtbl;
+    const float *dwtbl;  // INTERP convolvent: backward differences of wtbl (null: nearest neighbour)
     int ntiles, sector_len, n_ang, half, lds_wtbl, pair_base, pair_end;
     GrGatherGeom g;
     float L2m, dmax, rmax;
@@ -131,7 +132,8 @@ __device__ __forceinline__ void store_pairs(float2 *dst, const float2 (&v)[P]) {
 template <int ACCUM, int P>
 __device__ __forceinline__ void chunk(const float2 *spec, long row_off, int c0, int nj, int j0, int j1, float C,
                                       float S, float qa, float qb, const GrGatherGeom &g, const float *wtbl,
-                                      float2 (*stage)[P], int lane, float4 (&acc)[P], float2 (&part)[P]) {
+                                      const float *dwtbl, float2 (*stage)[P], int lane, float4 (&acc)[P],
+                                      float2 (&part)[P]) {
     if (lane < nj) {
         float2 v[P];
         load_pairs<P>(spec + (row_off + c0 + lane) * kSpecGroup, v);
@@ -140,7 +142,7 @@ __device__ __forceinline__ void chunk(const float2 *spec, long row_off, int c0, 
     __builtin_amdgcn_wave_barrier();
     for (int j = j0; j <= j1; j++) {
         float w;
-        if (gr_tap(j, C, S, qa, qb, g, wtbl, w)) {
+        if (dwtbl ? gr_tap_interp(j, C, S, qa, qb, g, wtbl, dwtbl, w) : gr_tap(j, C, S, qa, qb, g, wtbl, w)) {
             float2 v[P];
             load_pairs<P>(stage[j - c0], v);
 #pragma unroll
@@ -222,8 +224,14 @@ __device__ __forceinline__ void store_cells(const GridArgs &A, int pair0, int nv
 template <int ACCUM, int P>
 __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     extern __shared__ float smem[];
+    // the convolvent table (and its differences for INTERP) in LDS, then the
+    // waves' staging areas from lds_wtbl on
     float *s_wtbl = smem;
-    for (int i = threadIdx.x; i <= A.g.ltbl; i += blockDim.x) s_wtbl[i] = A.wtbl[i];
+    float *s_dwtbl = A.dwtbl ? smem + (A.g.ltbl + 1) : nullptr;
+    for (int i = threadIdx.x; i <= A.g.ltbl; i += blockDim.x) {
+        s_wtbl[i] = A.wtbl[i];
+        if (s_dwtbl) s_dwtbl[i] = A.dwtbl[i];
+    }
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -256,7 +264,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
         A, tx, ty, valid, pa - M2, pb - M2,
         [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
             chunk<ACCUM, P>(mirror ? base2 : base1, row_off, c0, nj, j0, j1, C, S, mirror ? ma : pa,
-                            mirror ? mb : pb, A.g, s_wtbl, stage, lane, acc, part);
+                            mirror ? mb : pb, A.g, s_wtbl, s_dwtbl, stage, lane, acc, part);
         },
         [&]() {
             if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
@@ -355,7 +363,9 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
         [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
             for (int j = j0; j <= j1; j++) {
                 float w;
-                if (gr_tap(j, C, S, mirror ? ma : pa, mirror ? mb : pb, A.g, A.wtbl, w)) {
+                const float qa = mirror ? ma : pa, qb = mirror ? mb : pb;
+                if (A.dwtbl ? gr_tap_interp(j, C, S, qa, qb, A.g, A.wtbl, A.dwtbl, w)
+                            : gr_tap(j, C, S, qa, qb, A.g, A.wtbl, w)) {
                     if (MODE == 1 && has_pend) put(aligned ? g0 + ng - 1 : n - 1, pend);
                     pend = make_uint2((unsigned)(row_off + j) | (mirror ? kGopMirror : 0u), __float_as_uint(w));
                     has_pend = true;
@@ -577,11 +587,13 @@ GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) 
     A.akey = t.akey;
     A.tiles = t.tiles;
     A.wtbl = t.wtbl;
+    A.dwtbl = t.dwtbl;
     A.ntiles = t.ntiles;
     A.sector_len = t.sector_len;
     A.n_ang = n_ang;
     A.half = (int)g.half;
-    A.lds_wtbl = (int)((ltbl + 1 + 3) & ~3L);  // keep the staging area 16-byte aligned
+    // wtbl (and dwtbl) in LDS; keep the staging area 16-byte aligned
+    A.lds_wtbl = (int)(((t.dwtbl ? 2 : 1) * (ltbl + 1) + 3) & ~3L);
     A.pair_base = 0;
     A.pair_end = 0;
     A.g.scale = g.scale;

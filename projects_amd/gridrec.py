@@ -37,7 +37,7 @@ class GrParams(C.Structure):
         ("R", C.c_float), ("X0", C.c_float), ("Y0", C.c_float), ("filter", C.c_char * 16),
         ("ltbl", C.c_long), ("pad", C.c_int), ("crop_off", C.c_long), ("crop_n", C.c_long),
         ("accum", C.c_int), ("tables", C.c_int), ("device", C.c_int), ("max_pairs", C.c_int),
-        ("grid_op", C.c_int), ("filter_fn", C.c_void_p), ("pswf", C.POINTER(GrPswf)),
+        ("grid_op", C.c_int), ("interp", C.c_int), ("filter_fn", C.c_void_p), ("pswf", C.POINTER(GrPswf)),
     ]
 
 
@@ -186,7 +186,7 @@ def make_params(n_ang: int, n_det: int, center: float, geom: int = 1, angles: Op
                 filter: str = "shlo", C_: float = 6.0, sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0,
                 X0: float = 0.0, Y0: float = 0.0, ltbl: int = 512, pad: str = "none",
                 crop: Optional[Tuple[int, int]] = None, accum: str = "compensated", tables: str = "reference",
-                device: int = 0, max_pairs: int = 0, grid_op: str = "auto"):
+                device: int = 0, max_pairs: int = 0, grid_op: str = "auto", interp: bool = False):
     """gr_params of one geometry, and the angle array it points to (keep it alive)."""
     ang = None
     ang_ptr = None
@@ -196,7 +196,7 @@ def make_params(n_ang: int, n_det: int, center: float, geom: int = 1, angles: Op
     crop_off, crop_n = crop if crop is not None else (0, 0)
     prm = GrParams(n_ang, n_det, geom, ang_ptr, center, C_, sampl, MaxPixSiz, R, X0, Y0, filter.encode()[:15], ltbl,
                    PAD[pad], crop_off, crop_n, ACCUM[accum], TABLES[tables], device, max_pairs,
-                   GRID_OP[grid_op], None, None)
+                   GRID_OP[grid_op], int(bool(interp)), None, None)
     return prm, ang
 
 
@@ -208,9 +208,10 @@ class Plan:
                  sampl: float = 1.0, MaxPixSiz: float = 1.0, R: float = 1.0, X0: float = 0.0, Y0: float = 0.0,
                  ltbl: int = 512, pad: str = "none", crop: Optional[Tuple[int, int]] = None,
                  accum: str = "compensated", tables: str = "reference", device: int = 0, max_pairs: int = 0,
-                 grid_op: str = "auto"):
+                 grid_op: str = "auto", interp: bool = False):
         self.params, self._angles = make_params(n_ang, n_det, center, geom, angles, filter, C_, sampl, MaxPixSiz, R,
-                                                X0, Y0, ltbl, pad, crop, accum, tables, device, max_pairs, grid_op)
+                                                X0, Y0, ltbl, pad, crop, accum, tables, device, max_pairs, grid_op,
+                                                interp)
         h = C.c_void_p()
         _check(lib().gr_plan_create(C.byref(self.params), C.byref(h)), "gr_plan_create")
         self._h = h

@@ -29,6 +29,7 @@ CASES = {
     "angle-list": dict(n_ang=len(_ANGLES), n_det=128, center=63.5, geom=0, angles=_ANGLES),
     "roi-offset": dict(n_ang=120, n_det=128, center=64.0, X0=0.1, Y0=-0.05),
     "hann-C4": dict(n_ang=75, n_det=160, center=81.0, filter="hann", C_=4.0),
+    "interp": dict(n_ang=90, n_det=128, center=63.5, interp=True),
 }
 
 
