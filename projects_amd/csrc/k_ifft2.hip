@@ -66,17 +66,27 @@ __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
     return (long)(x % 8u) * (ngroups / 8u) + x / 8u;
 }
 
-template <int MAXB>
+// ONE: one column per workgroup (the default), known at compile time -- the
+// column loop then folds away and the kernel needs 62 VGPRs instead of 112,
+// so LDS, not registers, sets the workgroups per CU.
+template <int MAXB, bool ONE>
 __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
     extern __shared__ float2 cols[];  // cw columns of gr_lds_len(M) slots
     const int p = blockIdx.y;
-    const long c0 = column_group(blockIdx.x, gridDim.x) * A.cw;
+    const int M = A.M, cw = ONE ? 1 : A.cw;
+    const long c0 = column_group(blockIdx.x, gridDim.x) * cw;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int M = A.M, cw = A.cw;
     const float2 *zp = A.z + (long)p * M * A.out_n;
-    for (long idx = tid; idx < (long)cw * M; idx += nthr) {
-        const long b = idx / cw, c = idx - b * cw;
-        cols[c * gr_lds_len(M) + gr_lds_idx((int)b)] = (c0 + c < A.out_n) ? zp[b * A.out_n + c0 + c] : make_float2(0.0f, 0.0f);
+    const int out_n = (int)A.out_n;
+    if (cw == 1) {
+        const float2 *zc = zp + c0;
+        for (int b = tid; b < M; b += nthr) cols[gr_lds_idx(b)] = c0 < out_n ? zc[(long)b * out_n] : make_float2(0.0f, 0.0f);
+    } else {
+        for (int idx = tid; idx < cw * M; idx += nthr) {  // cw * M <= 16384
+            const int b = idx / cw, c = idx - b * cw;
+            cols[c * gr_lds_len(M) + gr_lds_idx(b)] =
+                (c0 + c < out_n) ? zp[(long)b * out_n + c0 + c] : make_float2(0.0f, 0.0f);
+        }
     }
     __syncthreads();
     for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
@@ -89,12 +99,14 @@ __global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
         if (jj >= A.out_n) break;
         const float cu = A.winv[A.out_off + jj];
         const float2 *col = cols + c * gr_lds_len(M);
-        for (long kk = tid; kk < A.out_n; kk += nthr) {
-            const long k = A.out_off + kk;
+        float *r1 = o1 + jj * A.out_row_stride, *r2 = o2 ? o2 + jj * A.out_row_stride : nullptr;
+        const int off = (int)A.out_off, m02 = (int)A.M02;
+        for (int kk = tid; kk < out_n; kk += nthr) {
+            const int k = off + kk;
             const float corrn = cu * A.winv[k];  // grid.cpp:401
-            const float2 v = col[gr_lds_idx((int)corner_index(k, M, A.M02))];
-            o1[jj * A.out_row_stride + kk] = corrn * v.x;
-            if (o2) o2[jj * A.out_row_stride + kk] = corrn * v.y;
+            const float2 v = col[gr_lds_idx(k < m02 ? M - m02 + k : k - m02)];
+            r1[kk] = corrn * v.x;
+            if (r2) r2[kk] = corrn * v.y;
         }
     }
 }
@@ -155,14 +167,27 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     const int nthr = gr_fft_threads(g.M);
     const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(nthr);
     const size_t lds = sizeof(float2) * A.cw * gr_lds_len(g.M);
+#define GR_COLS(MB)                                                                      \
+    do {                                                                                 \
+        if (A.cw == 1) {                                                                 \
+            hipError_t e_ = gr_allow_lds(k_ifft_cols<MB, true>, lds);                    \
+            if (e_ != hipSuccess) return e_;                                             \
+            hipLaunchKernelGGL((k_ifft_cols<MB, true>), grid, block, lds, s, A);         \
+        } else {                                                                         \
+            hipError_t e_ = gr_allow_lds(k_ifft_cols<MB, false>, lds);                   \
+            if (e_ != hipSuccess) return e_;                                             \
+            hipLaunchKernelGGL((k_ifft_cols<MB, false>), grid, block, lds, s, A);        \
+        }                                                                                \
+    } while (0)
     switch (maxb_of(g.M, nthr)) {
     case 0:
-    case 1: { hipError_t e_ = gr_allow_lds(k_ifft_cols<1>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<1>, grid, block, lds, s, A); } break;
-    case 2: { hipError_t e_ = gr_allow_lds(k_ifft_cols<2>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<2>, grid, block, lds, s, A); } break;
-    case 4: { hipError_t e_ = gr_allow_lds(k_ifft_cols<4>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<4>, grid, block, lds, s, A); } break;
-    case 8: { hipError_t e_ = gr_allow_lds(k_ifft_cols<8>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<8>, grid, block, lds, s, A); } break;
-    case 16: { hipError_t e_ = gr_allow_lds(k_ifft_cols<16>, lds); if (e_ != hipSuccess) return e_; hipLaunchKernelGGL(k_ifft_cols<16>, grid, block, lds, s, A); } break;
+    case 1: { GR_COLS(1); } break;
+    case 2: { GR_COLS(2); } break;
+    case 4: { GR_COLS(4); } break;
+    case 8: { GR_COLS(8); } break;
+    case 16: { GR_COLS(16); } break;
     default: return hipErrorInvalidValue;
     }
+#undef GR_COLS
     return hipGetLastError();
 }
