@@ -231,6 +231,13 @@ int gr_ring_correct(float *sino, long slice_stride, long row_stride, int n_slice
 /* recon_free. */
 int gr_plan_destroy(gr_plan *plan);
 
+/* Build the plan's gather operator now, on `stream` (NULL: the null stream),
+ * as GR_GRIDOP_ON does at creation; synchronises that stream.  No effect on a
+ * plan that has one or whose GR_GRIDOP_AUTO build declined; GR_ENOMEM when it
+ * cannot be built.  No counterpart in the reference (grid.h): an extension for
+ * callers that choose when the build's few milliseconds are spent. */
+int gr_plan_build_grid_operator(gr_plan *plan, void *stream);
+
 /* Per-kernel HIP-event timing of gr_recon_device: an event pair around each
  * launch, resolved (waited for) by gr_get_timing. */
 int gr_set_timing(gr_plan *plan, int enable);
@@ -277,6 +284,8 @@ typedef struct gr_queue_info {
     long slices;            /* total slices of the batch */
     long graph_nodes;
     size_t workspace_bytes; /* shared per-stream workspaces */
+    int datasets_with_operator; /* datasets gridded from a precomputed gather operator */
+    size_t operator_bytes;      /* device memory those operators hold */
 } gr_queue_info;
 
 typedef struct gr_queue gr_queue;

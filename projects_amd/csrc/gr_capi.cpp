@@ -8,6 +8,7 @@
 // in chunks of max_pairs; nothing is allocated per call.  Plans are independent
 // objects: unlike the reference's globals, any number may coexist.
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -127,14 +128,24 @@ constexpr unsigned kGopHeavyRows = 64;
 // Build K2's gather operator on stream s: count pass, exclusive sum of the
 // per-tile entry counts on the host, fill pass.  Blocks the host until it is
 // built.  With `required` false a geometry whose operator would not fit (its
-// sample index range, its entry count, or a third of the free device memory)
-// is declined quietly and the plan keeps evaluating the geometry per launch.
-int build_gop(gr_plan *pl, hipStream_t s, bool required) {
+// sample index range, its entry count, a third of the free device memory, or
+// max_bytes) is declined quietly and the plan keeps evaluating the geometry
+// per launch.
+int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
     const auto t0 = std::chrono::steady_clock::now();
     const GrGeom &g = pl->g;
     const int n_ang = pl->prm.n_ang;
     auto decline = [&](const char *why) {
         pl->gop_state = -1;
+        (void)hipStreamSynchronize(s);
+        for (unsigned **pp : {&pl->d_gop_n, &pl->d_gop_off}) {
+            if (*pp) {
+                (void)hipFree(*pp);
+                *pp = nullptr;
+                pl->dev_bytes -= sizeof(unsigned) * pl->ntiles;
+            }
+        }
+        pl->gop_aligned_tiles = pl->gop_heavy = 0;
         return required ? fail(GR_ENOMEM, std::string("gather operator: ") + why) : GR_OK;
     };
     pl->gop_state = -1;  // until it is built: a failed build is not tried again
@@ -171,8 +182,8 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required) {
     GR_HIP(hipMemcpyAsync(pl->d_gop_n, cnt.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
     const size_t bytes = sizeof(uint2) * (size_t)std::max<unsigned long long>(total, 1);
     size_t free_b = 0, total_b = 0;
-    if (!required && (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b / 3))
-        return decline("larger than a third of the free device memory");
+    if (!required && (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b / 3 || bytes > max_bytes))
+        return decline("larger than a third of the free device memory or the caller's budget");
     if (hipMalloc((void **)&pl->d_gop, bytes) != hipSuccess) {
         pl->d_gop = nullptr;
         (void)hipGetLastError();
@@ -360,6 +371,11 @@ void gr_plan_bind_workspace(gr_plan *pl, void *base, int max_pairs) {
     pl->owns_workspace = false;
 }
 
+int gr_plan_build_operator(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
+    if (pl->gop_state != 0) return GR_OK;
+    return build_gop(pl, s, required, max_bytes);
+}
+
 extern "C" int gr_plan_create(const gr_params *params, gr_plan **plan_out) {
     return gr_plan_create_impl(params, plan_out, true);
 }
@@ -497,7 +513,7 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
     if (params->grid_op == GR_GRIDOP_ON) {
         hipStream_t bs;
         CK(plan_stream(pl, &bs));
-        CK(build_gop(pl, bs, true));
+        CK(build_gop(pl, bs, true, SIZE_MAX));
     }
 #undef CK
     *plan_out = pl;
@@ -564,7 +580,7 @@ int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long si
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         GR_HIP(hipStreamIsCapturing(s, &cs));
         if (cs == hipStreamCaptureStatusNone) {
-            const int rc = build_gop(pl, s, false);
+            const int rc = build_gop(pl, s, false, SIZE_MAX);
             if (rc) return rc;
         }
     }
@@ -747,6 +763,13 @@ int gr_recon_host(gr_plan *pl, const float *const *G1, const float *const *G2, f
         memcpy(S2[j], pl->h_out + (on + j) * on, sizeof(float) * on);
     }
     return GR_OK;
+}
+
+int gr_plan_build_grid_operator(gr_plan *pl, void *stream) {
+    if (!pl) return fail(GR_EINVAL, "null plan");
+    GR_HIP(hipSetDevice(pl->prm.device));
+    if (pl->gop_state != 0) return GR_OK;
+    return build_gop(pl, (hipStream_t)stream, true, SIZE_MAX);
 }
 
 int gr_plan_destroy(gr_plan *pl) {

@@ -88,3 +88,7 @@ size_t gr_plan_workspace_bytes(const gr_plan *plan, int max_pairs);
 // bytes, 256-byte aligned), owned by the caller.
 void gr_plan_bind_workspace(gr_plan *plan, void *base, int max_pairs);
 int gr_set_error(int code, const char *msg);
+// Build the plan's gather operator now, on stream s (once: a plan that has
+// one, or declined one, keeps that state).  With required false it declines
+// quietly beyond max_bytes or a third of the free device memory.
+int gr_plan_build_operator(gr_plan *plan, hipStream_t s, bool required, size_t max_bytes);

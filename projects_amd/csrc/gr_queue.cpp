@@ -118,13 +118,17 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     }
     // plans without workspaces
     q->plans.assign(n_datasets, nullptr);
+    std::vector<char> queue_op(n_datasets, 0);  // GR_GRIDOP_AUTO: the queue decides below
     for (int d = 0; d < n_datasets; d++) {
         gr_params p = datasets[d].params;
         p.device = device;
-        // a queue replays captured launches, where the automatic operator build
-        // never runs (it synchronises); it would only disturb the tuner's
-        // timings, so datasets build one only when asked to (GR_GRIDOP_ON)
-        if (p.grid_op == GR_GRIDOP_AUTO) p.grid_op = GR_GRIDOP_OFF;
+        // the plan's own automatic build must not run: it synchronises, which a
+        // captured graph cannot, and would change kernels between the tuner's
+        // trials; the queue builds AUTO datasets' operators itself
+        if (p.grid_op == GR_GRIDOP_AUTO) {
+            p.grid_op = GR_GRIDOP_OFF;
+            queue_op[d] = 1;
+        }
         if (datasets[d].n_slices < 1 || !datasets[d].sino || !datasets[d].out) {
             free_queue(q);
             return gr_set_error(GR_EINVAL, "dataset without slices or buffers");
@@ -148,6 +152,39 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         q->events.push_back(e);
     }
     hipStream_t s0 = q->streams[0];
+
+    // gather operators: a queue is built once and replayed, so a dataset's
+    // operator pays off from the first replay on.  Within a budget -- half of
+    // the device memory left after a generous estimate of the workspaces --
+    // datasets are served in order of slices (the operator's benefit per byte
+    // grows with the pairs it serves), then of size.
+    {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        size_t ws_est = 0;
+        for (int d = 0; d < n_datasets; d++)
+            ws_est = std::max(ws_est, gr_plan_workspace_bytes(q->plans[d], std::min((q->ds[d].n_slices + 1) / 2, 8)));
+        ws_est *= (size_t)(n_streams + 1);
+        size_t budget = free_b > ws_est ? (free_b - ws_est) / 2 : 0;
+        std::vector<int> order(n_datasets);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (q->ds[a].n_slices != q->ds[b].n_slices) return q->ds[a].n_slices > q->ds[b].n_slices;
+            const GrGeom &ga = q->plans[a]->g, &gb = q->plans[b]->g;
+            return (long)q->plans[a]->prm.n_ang * ga.half > (long)q->plans[b]->prm.n_ang * gb.half;
+        });
+        for (int d : order) {
+            if (!queue_op[d] || budget == 0) continue;
+            QCK(gr_plan_build_operator(q->plans[d], s0, false, budget));
+            const size_t b = q->plans[d]->gop_bytes;
+            budget = b < budget ? budget - b : 0;
+        }
+        for (int d = 0; d < n_datasets; d++) {
+            if (q->plans[d]->gop_state != 1) continue;
+            q->info.datasets_with_operator++;
+            q->info.operator_bytes += q->plans[d]->gop_bytes;
+        }
+    }
 
     // size classes and the autotuner
     std::map<std::tuple<long, long, long>, std::vector<int>> classes;

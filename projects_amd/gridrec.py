@@ -80,6 +80,7 @@ class GrQueueInfo(C.Structure):
     _fields_ = [
         ("n_datasets", C.c_int), ("n_streams", C.c_int), ("size_classes", C.c_int), ("tuned_candidates", C.c_int),
         ("slices", C.c_long), ("graph_nodes", C.c_long), ("workspace_bytes", C.c_size_t),
+        ("datasets_with_operator", C.c_int), ("operator_bytes", C.c_size_t),
     ]
 
 
@@ -100,6 +101,7 @@ _SIGS = {
     "gr_ring_correct": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
                                   C.c_void_p]),
     "gr_plan_destroy": (C.c_int, [C.c_void_p]),
+    "gr_plan_build_grid_operator": (C.c_int, [C.c_void_p, C.c_void_p]),
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
     "gr_phantom_sinograms": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_int,
@@ -225,6 +227,11 @@ class Plan:
         inf = GrInfo()
         _check(lib().gr_plan_info(self._h, C.byref(inf)), "gr_plan_info")
         return {f: getattr(inf, f) for f, _ in GrInfo._fields_}
+
+    def build_operator(self, stream=None):
+        """Build the gather operator now (gr_plan_build_grid_operator)."""
+        _check(lib().gr_plan_build_grid_operator(self._h, C.c_void_p(_stream_handle(stream))),
+               "gr_plan_build_grid_operator")
 
     def close(self):
         if getattr(self, "_h", None):

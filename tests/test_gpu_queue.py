@@ -2,8 +2,9 @@
 
 The queue's images must be bitwise those of a plain per-dataset plan
 (gr_recon_device with default settings) wherever the tuner kept the default
-FFT radix: slice pairs per launch, the column-FFT width and the gridding
-workgroup size change no sum order.  Where it chose radix 4 the FFTs add in
+FFT radix: slice pairs per launch, the column-FFT width, the gridding
+workgroup size and the gather operator the queue builds for its datasets
+change no sum order.  Where it chose radix 4 the FFTs add in
 another order and the images agree to FP32 rounding (rel-L2 <= 1e-6).  The
 graph is replayed several times with eager work of the program between
 replays, and every replay is checked against the eager result.
@@ -68,6 +69,8 @@ def test_queue_matches_plain_plans_and_replays():
     info = q.info
     print("queue info", info, [q.settings(d) for d in range(10)], flush=True)
     assert info["n_datasets"] == 10 and info["slices"] == sum(s[1].shape[0] for s in sets)
+    # small datasets: every one fits the operator budget
+    assert info["datasets_with_operator"] == 10 and info["operator_bytes"] > 0
     for d, (_, _, out) in enumerate(sets):  # create ran the batch once
         assert same_as_plain(q, d, out, ref[d]), d
     first = [out.cpu() for _, _, out in sets]
@@ -94,9 +97,12 @@ def test_queue_matches_plain_plans_and_replays():
 
 
 def test_queue_without_autotune_and_single_stream():
-    sets = uneven_batch(12, 5, slices=(2, 4))
+    """One stream, default settings, and datasets that refuse the gather
+    operator (grid_op off): the geometry is evaluated per launch."""
+    sets = [(dict(kw, grid_op="off"), sino, out) for kw, sino, out in uneven_batch(12, 5, slices=(2, 4))]
     ref = plain_results(sets)
     q = gr.Queue(sets, n_streams=1, autotune=False)
+    assert q.info["datasets_with_operator"] == 0
     for _, _, out in sets:
         out.fill_(float("nan"))
     q.run()
