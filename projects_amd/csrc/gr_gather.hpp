@@ -193,12 +193,3 @@ __host__ __device__ __forceinline__ void gr_candidates_r(float du, float dv, flo
     if (d0 <= d1) { jd0 = (int)d0; jd1 = (int)d1; }
     if (m0 <= m1) { jm0 = (int)m0; jm1 = (int)m1; }
 }
-
-// Kahan compensated summation: c carries the rounding error of the running
-// sum s with opposite sign; the total is s - c.
-__host__ __device__ __forceinline__ void gr_kahan(float &s, float &c, float x) {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
-}

@@ -21,21 +21,29 @@
 // angle of the window is then tested exactly (perpendicular distance against
 // the tile's half-width in the line's normal direction) and skipped when its
 // line misses the tile.  Central tiles scan every angle; tiles beyond the
-// largest sample radius are written as zeros.  Tiles are listed nearest to the
-// centre first, so the heaviest work is dispatched first.
+// largest sample radius hold only zeros and are not written (the row
+// transform reads them as zero, gr_tile_active).  The tile list is cut into
+// eight sectors of polar angle, one per XCD, each nearest to the centre
+// first, so the heaviest work is dispatched first and each XCD's L2 holds the
+// samples of its own sector.
 //
 // Per angle, the samples that can reach the tile form one contiguous range of
 // j for the direct term and one for the mirrored term; the wave stages the P
 // pairs' spectra of that range in its own LDS slot (64 samples at a time) and
 // each lane then runs its own exact candidate range against LDS.
 //
-// Accumulation: GR_ACCUM_PLAIN adds each tap's float product in float, in
-// the reference's order.  GR_ACCUM_COMPENSATED sums the taps of up to
+// Accumulation: GR_ACCUM_PLAIN adds each tap's float product in float, in a
+// fixed order (the tile's sample lines by sorted angle key, per line the
+// direct samples, then the mirrored ones).  GR_ACCUM_COMPENSATED sums the taps of up to
 // kMergeAngles consecutive sample lines (a few taps each) with fused
 // multiply-adds into a partial, and adds each partial into the cell's running
 // sum with an error-free TwoSum (Knuth), carrying the rounding errors in a
 // second float: the cell's value is the sum plus the carried error, rounded
 // once.
+//
+// Two kernels: k_grid derives the taps from the geometry per launch;
+// k_grid_op streams them from the plan's precomputed gather operator
+// (k_gop_build, below), bitwise the same.
 //
 // Output: Ht[pair][b][a] = H[a][b] (transposed, so K3's rows run over a).
 #include "gr_fft.hpp"
