@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--learned-seconds", type=float, default=90.0, help="training time budget")
     ap.add_argument("--learned-steps", type=int, default=0, help="training step cap (0 = time budget only)")
     ap.add_argument("--learned-slices", type=int, default=256, help="inference slices (2048^2, 150 angles)")
+    ap.add_argument("--learned-truth-slices", type=int, default=4,
+                    help="inference slices also compared with the analytic phantom")
     ap.add_argument("--learned-threshold", type=float, default=0.5,
                     help="fallback when ||residual|| / ||image|| exceeds this")
     ap.add_argument("--precision-slices", type=int, default=256)
@@ -420,6 +422,19 @@ def bench_learned(a, gr, torch):
     plain, _, t1500, _ = outs[1500]
     sparse, corr, t150, tcnn = outs[150]
     _, used = net.correct(sparse[0:16].contiguous())
+    # against the phantom itself: the analytic image on the window's pixel
+    # grid ([-1, 1]^2, density = reconstruction * n/2), inside the 0.95 disk
+    progress("learned: PSNR against the analytic phantoms")
+    from projects_amd import phantom as Ph
+    cc = (np.arange(n) - (n - 1) / 2.0) / (n / 2.0)
+    disk = np.add.outer(cc * cc, cc * cc) < 0.95 ** 2
+    truth_ps = {"plain1500": [], "plain150": [], "learned150": []}
+    for i in range(a.learned_truth_slices):
+        t = Ph.phantom_image(a.seed + 6, i, 20, n, n_tex=64, tex_amp=0.15, tex_fmax=24.0)[disk]
+        rngt = t.max() - t.min()
+        for key, img in (("plain1500", plain[i]), ("plain150", sparse[i]), ("learned150", corr[i])):
+            r = img.double().cpu().numpy()[disk] * (n / 2.0)
+            truth_ps[key].append(10.0 * np.log10(rngt * rngt / np.mean((r - t) ** 2)))
     ps_sparse = [Lrn.psnr(sparse[i], plain[i]) for i in range(ns)]
     ps_learned = [Lrn.psnr(corr[i], plain[i]) for i in range(ns)]
     rl = lambda u, v: float(torch.linalg.norm((u - v).double()) / torch.linalg.norm(v.double()))
@@ -435,6 +450,10 @@ def bench_learned(a, gr, torch):
         "psnr_learned150_vs_plain1500": round(float(np.mean(ps_learned)), 2),
         "rel_l2_plain150_vs_plain1500": float(f"{np.mean([rl(sparse[i], plain[i]) for i in range(ns)]):.4e}"),
         "rel_l2_learned150_vs_plain1500": float(f"{np.mean([rl(corr[i], plain[i]) for i in range(ns)]):.4e}"),
+        "psnr_vs_phantom": {k: round(float(np.mean(v)), 2) for k, v in truth_ps.items()},
+        "psnr_vs_phantom_slices": a.learned_truth_slices,
+        "learned150_minus_plain1500_dB_vs_phantom": round(float(np.mean(truth_ps["learned150"]) -
+                                                            np.mean(truth_ps["plain1500"])), 2),
         "fallback_slices_of_first16": int((~used).sum()),
         "threshold": a.learned_threshold,
     }
