@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--cpu-pairs", type=int, default=0, help="pairs in the CPU baseline sample (0 = one per thread)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--h2h-slices", type=int, default=256,
+                    help="slices timed host-to-host through gr_exec_pairs (0: skip)")
     ap.add_argument("--workload", default="all", choices=["all", "plain", "batched", "precision", "learned"],
                     help="all: the plain headline line with the batched queue, the precision study and the "
                          "learned sparse-angle path after it")
@@ -619,6 +621,23 @@ def main():
             q1, q2 = O.ref_recon(g, p1, p2)
             acc["rel_l2_vs_reference_build"] = max(O.rel_l2(o[0], q1[c, c]), O.rel_l2(o[1], q2[c, c]))
         result["accuracy"] = {k: float(f"{v:.3e}") for k, v in acc.items()}
+
+    # PCIe-inclusive rate: the reference's own boundary hands over host rows;
+    # gr_exec_pairs uploads, reconstructs and downloads in a pipeline
+    if a.h2h_slices > 0:
+        progress("plain: host-to-host (gr_exec_pairs)")
+        k = min(a.h2h_slices, ns)
+        h_sino = sino[0:k].cpu().numpy()
+        h_out = np.empty((k, out_n, out_n), np.float32)
+        plan.exec_pairs(h_sino[0:16], h_out[0:16])  # staging allocated
+        t0 = time.perf_counter()
+        plan.exec_pairs(h_sino, h_out)
+        hdt = time.perf_counter() - t0
+        same = bool(np.array_equal(h_out[0:2], out[0:2].cpu().numpy()))
+        result["host_to_host"] = {"slices": k, "slices_s": round(k / hdt, 1), "bitwise_equal_to_device_path": same,
+                                  "what": "host sinogram rows in, host image rows out: PCIe both ways, pinned "
+                                          "staging, row packing on 16 host threads (gr_exec_pairs)"}
+        del h_sino, h_out
 
     # CPU baseline: the oracle's FP32 restatement on this host's cores, bounded sample
     if not a.no_cpu:

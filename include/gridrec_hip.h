@@ -173,6 +173,18 @@ int gr_plan_tables(const gr_plan *plan, float *sine, float *cose, float *filphas
 int gr_recon_host(gr_plan *plan, const float *const *G1, const float *const *G2,
                   float *const *S1, float *const *S2);
 
+/* do_recon for npairs slice pairs on host rows -- the reference's own
+ * host-buffer boundary for a whole batch: pair p's rows are G1[p*n_ang + n]
+ * and G2[p*n_ang + n] (n < n_ang, n_det floats each, any addresses), its
+ * images S1[p*out_n + j] and S2[p*out_n + j] (j < out_n, out_n floats each).
+ * Pipelined through two pinned staging slots of max_pairs pairs (allocated on
+ * the first host-row call): uploads and downloads run on two streams of the
+ * plan's while `stream` reconstructs, and host threads pack and unpack rows
+ * meanwhile.  Synchronous on return.  Replaces the per-pair do_recon loop of
+ * the callers (Multitomo_unit.txt:823-856). */
+int gr_exec_pairs(gr_plan *plan, const float *const *G1, const float *const *G2,
+                  float *const *S1, float *const *S2, int npairs, void *stream);
+
 /* do_recon for n_slices slices resident in device memory.  Slice s starts at
  * sino + s*sino_slice_stride and holds n_ang rows of n_det floats spaced
  * sino_row_stride floats apart.  Slices (2p, 2p+1) form pair p and share one

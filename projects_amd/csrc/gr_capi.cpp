@@ -19,6 +19,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -233,8 +234,8 @@ std::vector<unsigned> sector_tiles(int nt, int ts, long M) {
 void free_plan(gr_plan *pl) {
     if (!pl) return;
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
-                    pl->d_aord, pl->d_ang, pl->d_in, pl->d_out, pl->d_gop, pl->d_gop_n, pl->d_gop_off,
-                    pl->d_dwtbl};
+                    pl->d_aord, pl->d_ang, pl->d_in[0], pl->d_in[1], pl->d_out[0], pl->d_out[1], pl->d_gop,
+                    pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -245,8 +246,14 @@ void free_plan(gr_plan *pl) {
         for (void *p : ws)
             if (p) (void)hipFree(p);
     }
-    if (pl->h_in) (void)hipHostFree(pl->h_in);
-    if (pl->h_out) (void)hipHostFree(pl->h_out);
+    for (int b = 0; b < 2; b++) {
+        if (pl->h_in[b]) (void)hipHostFree(pl->h_in[b]);
+        if (pl->h_out[b]) (void)hipHostFree(pl->h_out[b]);
+        for (hipEvent_t e : {pl->ev_up[b], pl->ev_done[b], pl->ev_down[b]})
+            if (e) (void)hipEventDestroy(e);
+    }
+    if (pl->up_stream) (void)hipStreamDestroy(pl->up_stream);
+    if (pl->down_stream) (void)hipStreamDestroy(pl->down_stream);
     for (auto &q : pl->pending) {
         (void)hipEventDestroy(q.e0);
         (void)hipEventDestroy(q.e1);
@@ -616,6 +623,107 @@ int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long si
     return GR_OK;
 }
 
+// Run fn(lo, hi) over [0, n) split across up to 16 host threads.
+template <typename F>
+void parallel_rows(long n, F &&fn) {
+    const long nt = std::max(1L, std::min<long>(16, std::min<long>((long)std::thread::hardware_concurrency(), n / 64)));
+    if (nt <= 1) {
+        fn(0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long t = 0; t < nt; t++) th.emplace_back([&, t] { fn(n * t / nt, n * (t + 1) / nt); });
+    for (auto &x : th) x.join();
+}
+
+// Host rows in, host rows out, for npairs pairs, in chunks of stage_pairs
+// pairs through two staging slots: while the GPU uploads, reconstructs and
+// downloads chunk c (upload stream, compute stream `cs`, download stream,
+// ordered by events; uploads and downloads use the two DMA directions at
+// once), the host unpacks chunk c-1's images and packs chunk c+1's rows into
+// the other slot.  Synchronous on return.
+int exec_pairs(gr_plan *pl, const float *const *G1, const float *const *G2, float *const *S1, float *const *S2,
+               int npairs, hipStream_t cs) {
+    const GrGeom &g = pl->g;
+    const int n_ang = pl->prm.n_ang, nd = g.n_det_in;
+    const long on = g.out_n;
+    const size_t in_pp = (size_t)2 * n_ang * nd, out_pp = (size_t)2 * on * on;  // floats per pair
+    if (!pl->stage_pairs) {
+        const int sp = pl->max_pairs;
+        for (int b = 0; b < 2; b++) {
+            GR_HIP(hipHostMalloc((void **)&pl->h_in[b], sizeof(float) * in_pp * sp));
+            GR_HIP(hipHostMalloc((void **)&pl->h_out[b], sizeof(float) * out_pp * sp));
+            int rc = dev_alloc(pl, &pl->d_in[b], in_pp * sp);
+            if (rc) return rc;
+            rc = dev_alloc(pl, &pl->d_out[b], out_pp * sp);
+            if (rc) return rc;
+            GR_HIP(hipEventCreateWithFlags(&pl->ev_up[b], hipEventDisableTiming));
+            GR_HIP(hipEventCreateWithFlags(&pl->ev_done[b], hipEventDisableTiming));
+            GR_HIP(hipEventCreateWithFlags(&pl->ev_down[b], hipEventDisableTiming));
+        }
+        GR_HIP(hipStreamCreateWithFlags(&pl->up_stream, hipStreamNonBlocking));
+        GR_HIP(hipStreamCreateWithFlags(&pl->down_stream, hipStreamNonBlocking));
+        pl->stage_pairs = sp;
+    }
+    const int sp = pl->stage_pairs, nch = (npairs + sp - 1) / sp;
+    hipStream_t up = pl->up_stream, down = pl->down_stream;
+    auto pairs_of = [&](int c) { return std::min(sp, npairs - c * sp); };
+    auto pack = [&](int c, int b) {  // slice 2q <- G1 rows of pair p0+q, slice 2q+1 <- G2 rows
+        const long p0 = (long)c * sp, rows = 2L * pairs_of(c) * n_ang;
+        float *dst = pl->h_in[b];
+        parallel_rows(rows, [&](long lo, long hi) {
+            for (long r = lo; r < hi; r++) {
+                const long sl = r / n_ang, n = r - sl * n_ang, pair = p0 + sl / 2;
+                const float *src = (sl & 1 ? G2 : G1)[pair * n_ang + n];
+                memcpy(dst + r * nd, src, sizeof(float) * nd);
+            }
+        });
+    };
+    auto unpack = [&](int c, int b) {
+        const long p0 = (long)c * sp, rows = 2L * pairs_of(c) * on;
+        const float *src = pl->h_out[b];
+        parallel_rows(rows, [&](long lo, long hi) {
+            for (long r = lo; r < hi; r++) {
+                const long sl = r / on, j = r - sl * on, pair = p0 + sl / 2;
+                memcpy((sl & 1 ? S2 : S1)[pair * on + j], src + r * on, sizeof(float) * on);
+            }
+        });
+    };
+    // slot b's chunk c: upload (after chunk c-2's gridrec has read d_in[b]),
+    // gridrec, download; chunk c-2's download, which d_out[b] also waits for,
+    // the host has already waited for
+    auto launch = [&](int c, int b) -> int {
+        const int np = pairs_of(c);
+        if (c >= 2) GR_HIP(hipStreamWaitEvent(up, pl->ev_done[b], 0));
+        GR_HIP(hipMemcpyAsync(pl->d_in[b], pl->h_in[b], sizeof(float) * in_pp * np, hipMemcpyHostToDevice, up));
+        GR_HIP(hipEventRecord(pl->ev_up[b], up));
+        GR_HIP(hipStreamWaitEvent(cs, pl->ev_up[b], 0));
+        const int rc = recon_chunks(pl, pl->d_in[b], (long)n_ang * nd, nd, nullptr, pl->d_out[b], on * on, on,
+                                    2 * np, cs);
+        if (rc) return rc;
+        GR_HIP(hipEventRecord(pl->ev_done[b], cs));
+        GR_HIP(hipStreamWaitEvent(down, pl->ev_done[b], 0));
+        GR_HIP(hipMemcpyAsync(pl->h_out[b], pl->d_out[b], sizeof(float) * out_pp * np, hipMemcpyDeviceToHost, down));
+        GR_HIP(hipEventRecord(pl->ev_down[b], down));
+        return GR_OK;
+    };
+    for (int c = 0; c < nch; c++) {
+        const int b = c & 1;
+        if (c >= 2) {  // slot b last held chunk c-2: its download (and so its upload) is done
+            GR_HIP(hipEventSynchronize(pl->ev_down[b]));
+            unpack(c - 2, b);
+        }
+        pack(c, b);
+        const int rc = launch(c, b);
+        if (rc) return rc;
+    }
+    for (int c = std::max(0, nch - 2); c < nch; c++) {
+        GR_HIP(hipEventSynchronize(pl->ev_down[c & 1]));
+        unpack(c, c & 1);
+    }
+    return GR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -733,36 +841,20 @@ int gr_recon_device_raw(gr_plan *pl, const gr_preproc *pre, float *out, long out
 int gr_recon_host(gr_plan *pl, const float *const *G1, const float *const *G2, float *const *S1,
                   float *const *S2) {
     if (!pl || !G1 || !G2 || !S1 || !S2) return fail(GR_EINVAL, "null argument");
-    const GrGeom &g = pl->g;
-    const int n_ang = pl->prm.n_ang, nd = g.n_det_in;
-    const long on = g.out_n;
-    const size_t in_n = (size_t)2 * n_ang * nd, out_n = (size_t)2 * on * on;
     GR_HIP(hipSetDevice(pl->prm.device));
-    if (!pl->h_in) {
-        GR_HIP(hipHostMalloc((void **)&pl->h_in, sizeof(float) * in_n));
-        GR_HIP(hipHostMalloc((void **)&pl->h_out, sizeof(float) * out_n));
-        int rc = dev_alloc(pl, &pl->d_in, in_n);
-        if (rc) return rc;
-        rc = dev_alloc(pl, &pl->d_out, out_n);
-        if (rc) return rc;
-    }
-    for (int n = 0; n < n_ang; n++) {
-        memcpy(pl->h_in + (size_t)n * nd, G1[n], sizeof(float) * nd);
-        memcpy(pl->h_in + (size_t)(n_ang + n) * nd, G2[n], sizeof(float) * nd);
-    }
     hipStream_t st;
-    int rc = plan_stream(pl, &st);
+    const int rc = plan_stream(pl, &st);
     if (rc) return rc;
-    GR_HIP(hipMemcpyAsync(pl->d_in, pl->h_in, sizeof(float) * in_n, hipMemcpyHostToDevice, st));
-    rc = gr_recon_device(pl, pl->d_in, (long)n_ang * nd, nd, pl->d_out, on * on, on, 2, st);
-    if (rc) return rc;
-    GR_HIP(hipMemcpyAsync(pl->h_out, pl->d_out, sizeof(float) * out_n, hipMemcpyDeviceToHost, st));
-    GR_HIP(hipStreamSynchronize(st));
-    for (long j = 0; j < on; j++) {
-        memcpy(S1[j], pl->h_out + j * on, sizeof(float) * on);
-        memcpy(S2[j], pl->h_out + (on + j) * on, sizeof(float) * on);
-    }
-    return GR_OK;
+    return exec_pairs(pl, G1, G2, S1, S2, 1, st);
+}
+
+int gr_exec_pairs(gr_plan *pl, const float *const *G1, const float *const *G2, float *const *S1,
+                  float *const *S2, int npairs, void *stream) {
+    if (!pl || !G1 || !G2 || !S1 || !S2) return fail(GR_EINVAL, "null argument");
+    if (npairs < 0) return fail(GR_EINVAL, "negative pair count");
+    if (npairs == 0) return GR_OK;
+    GR_HIP(hipSetDevice(pl->prm.device));
+    return exec_pairs(pl, G1, G2, S1, S2, npairs, (hipStream_t)stream);
 }
 
 int gr_plan_build_grid_operator(gr_plan *pl, void *stream) {

@@ -39,8 +39,15 @@ struct gr_plan {
     size_t dev_bytes = 0;
     // launch tuning (0 = automatic)
     int cols_per_block = 0;
-    // host staging for gr_recon_host
-    float *h_in = nullptr, *h_out = nullptr, *d_in = nullptr, *d_out = nullptr;
+    // host-row entry points (gr_exec_pairs, gr_recon_host): two staging slots
+    // of stage_pairs pairs each, pinned on the host and on the device, an
+    // upload and a download stream beside the compute stream, and per slot the
+    // events that order upload -> gridrec -> download
+    int stage_pairs = 0;
+    float *h_in[2] = {nullptr, nullptr}, *h_out[2] = {nullptr, nullptr};
+    float *d_in[2] = {nullptr, nullptr}, *d_out[2] = {nullptr, nullptr};
+    hipStream_t up_stream = nullptr, down_stream = nullptr;  // host-to-device, device-to-host
+    hipEvent_t ev_up[2] = {nullptr, nullptr}, ev_done[2] = {nullptr, nullptr}, ev_down[2] = {nullptr, nullptr};
     // timing: event pairs recorded around launches, resolved in gr_get_timing
     int timing = 0;
     gr_timing t{};

@@ -92,6 +92,7 @@ _SIGS = {
     "gr_plan_info": (C.c_int, [C.c_void_p, C.POINTER(GrInfo)]),
     "gr_plan_tables": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_float)] * 5),
     "gr_recon_host": (C.c_int, [C.c_void_p] + [C.POINTER(C.POINTER(C.c_float))] * 4),
+    "gr_exec_pairs": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_void_p]),
     "gr_recon_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long, C.c_long,
                                   C.c_int, C.c_void_p]),
     "gr_recon_device_raw": (C.c_int, [C.c_void_p, C.POINTER(GrPreproc), C.c_void_p, C.c_long, C.c_long, C.c_int,
@@ -329,6 +330,35 @@ class Plan:
 
         _check(lib().gr_recon_host(self._h, rows(g1), rows(g2), rows(s1), rows(s2)), "gr_recon_host")
         return s1, s2
+
+    def exec_pairs(self, sino: np.ndarray, out: np.ndarray = None, stream=None) -> np.ndarray:
+        """do_recon for a batch on host memory (gr_exec_pairs): sino is a float32
+        [n_slices, n_ang, >=n_det] array (rows may be strided), slices (2p,
+        2p+1) forming pair p; returns (or fills) float32 [n_slices, out_n, out_n].
+        An odd last slice is paired with zero rows."""
+        assert sino.dtype == np.float32 and sino.ndim == 3 and sino.strides[2] == 4
+        assert sino.shape[1] == self.n_ang and sino.shape[2] >= self.n_det
+        ns, on = sino.shape[0], self.out_n
+        if out is None:
+            out = np.empty((ns, on, on), np.float32)
+        assert out.dtype == np.float32 and out.shape[0] == ns and out.strides[2] == 4
+        assert out.shape[1] >= on and out.shape[2] >= on
+        npairs = (ns + 1) // 2
+        zeros = np.zeros((self.n_ang, sino.shape[2]), np.float32)
+        spare = np.empty((on, on), np.float32)
+
+        def ptrs(views):  # row addresses of a list of 2-D row-major views, in order
+            return np.concatenate([v.ctypes.data + np.arange(v.shape[0], dtype=np.uint64) * np.uint64(v.strides[0])
+                                   for v in views]).astype(np.uint64)
+
+        g1 = ptrs([sino[2 * p] for p in range(npairs)])
+        g2 = ptrs([sino[2 * p + 1] if 2 * p + 1 < ns else zeros for p in range(npairs)])
+        s1 = ptrs([out[2 * p, :on] for p in range(npairs)])
+        s2 = ptrs([out[2 * p + 1, :on] if 2 * p + 1 < ns else spare for p in range(npairs)])
+        vp = lambda a: C.c_void_p(a.ctypes.data)
+        _check(lib().gr_exec_pairs(self._h, vp(g1), vp(g2), vp(s1), vp(s2), npairs,
+                                   C.c_void_p(_stream_handle(stream))), "gr_exec_pairs")
+        return out
 
     def set_timing(self, on: bool):
         _check(lib().gr_set_timing(self._h, int(on)), "gr_set_timing")

@@ -301,3 +301,21 @@ def test_interp_convolvent_matches_oracle(accum):
     n1, _ = O.recon_f32(O.Geometry(n_ang=120, n_det=256, center=127.25), sino[0], sino[1])
     assert O.rel_l2(out[0], n1) > 10 * O.rel_l2(out[0], r1)
     p.close()
+
+
+def test_exec_pairs_host_rows_match_device_path():
+    """gr_exec_pairs (host rows in, host rows out, two staging slots): 9 slices
+    through a plan of 2 pairs per launch -- three chunks, both slots reused,
+    an odd last slice -- from rows with a stride wider than the detector,
+    bitwise the device path's images."""
+    g = O.Geometry(n_ang=90, n_det=128, center=63.75)
+    rng = np.random.default_rng(31)
+    wide = rng.standard_normal((9, 90, 140)).astype(np.float32)
+    sino = wide[:, :, :128]
+    p = plan_for(g, max_pairs=2)
+    host = p.exec_pairs(sino)
+    dev = p.recon(torch.from_numpy(np.ascontiguousarray(sino)).cuda()).cpu().numpy()
+    assert np.array_equal(host, dev)
+    again = p.exec_pairs(sino, out=np.full_like(host, np.nan))  # staging reused
+    assert np.array_equal(again, dev)
+    p.close()
