@@ -24,8 +24,8 @@
 // transposed in LDS.  Blocks loop over tiles and write per-block partial sums
 // that k_reduce adds in a fixed order: results are bitwise repeatable.
 //
-// The 1 -> 32 and 32 -> 1 layers (288 multiply-adds per pixel) run on the
-// vector ALUs.
+// The 1 -> 32 and 32 -> 1 layers run on MFMA as well: K over the 9 taps of
+// the single input channel, and W5 in row 0 of A.
 #include <math.h>
 #include <stdint.h>
 
@@ -70,20 +70,60 @@ __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(
 __device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
 __device__ __forceinline__ bool positive_bf16(uint16_t h) { return !(h & 0x8000u) && h != 0; }
 
-// s + sum over ci = 0..31 (in order) of w[ci*9 + t] * px[ci], px one pixel's
-// 32 bf16 channels in LDS (16-byte aligned), read 8 channels at a time.
-__device__ __forceinline__ float dot_tap(const uint16_t *px, const float *w, int t, float s) {
+__device__ __forceinline__ __bf16 bf16_of(float f) { return __builtin_bit_cast(__bf16, to_bf16(f)); }
+
+// The 1 -> 32 and 32 -> 1 layers on MFMA too (v_mfma_f32_16x16x32_bf16, the
+// lane layout of the 32 -> 32 layers: A row = lane & 15, B column = lane & 15,
+// k = 8 * (lane >> 4) + e; D column = lane & 15, rows 4 * (lane >> 4) + e).
+//
+// 1 -> 32: K runs over the 9 taps (k >= 9 zero), so one MFMA per 16 output
+// channels gives D[co][pixel] = sum_t W1[co][t] x(pixel + t).  w1 holds the
+// bf16 values of W1 as floats, [co * 9 + t].
+__device__ __forceinline__ void conv_in_frag(const float *w1, int lane, bf16x8 (&A)[2]) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const uint4 v = *reinterpret_cast<const uint4 *>(px + 8 * q);
-        const uint32_t u[4] = {v.x, v.y, v.z, v.w};
+    for (int nb = 0; nb < 2; nb++)
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            const float a = __uint_as_float(e & 1 ? (u[e >> 1] & 0xffff0000u) : (u[e >> 1] << 16));
-            s += w[(8 * q + e) * 9 + t] * a;
+            const int k = 8 * (lane >> 4) + e;
+            A[nb][e] = bf16_of(k < 9 ? w1[(nb * 16 + (lane & 15)) * 9 + k] : 0.0f);
         }
+}
+
+// B of the 1 -> 32 layer for the pixel whose 3 x 3 neighbourhood starts at
+// xs[r * pitch + c] (bf16 values held as floats).
+__device__ __forceinline__ bf16x8 conv_in_b(const float *xs, int pitch, int r, int c, int lane) {
+    bf16x8 B;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * (lane >> 4) + e;
+        B[e] = bf16_of(k < 9 ? xs[(r + k / 3) * pitch + c + k % 3] : 0.0f);
     }
-    return s;
+    return B;
+}
+
+// 32 -> 1: nine MFMAs (one per tap, K over the 32 input channels) with W5 in
+// row 0 of A and zeros in rows 1..15; the result D[0][pixel] sits in element 0
+// of lanes 0..15.  w5 holds the bf16 values of W5 as floats, [ci * 9 + t].
+__device__ __forceinline__ void conv_out_frag(const float *w5, int lane, bf16x8 (&A)[9]) {
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+            A[t][e] = bf16_of((lane & 15) == 0 ? w5[(8 * (lane >> 4) + e) * 9 + t] : 0.0f);
+}
+
+// D of the 32 -> 1 layer for the pixel whose neighbourhood starts at record
+// (r, c) of an LDS region of `pitch` records (PSTR bf16 each).
+__device__ __forceinline__ floatx4 conv_out_mfma(const uint16_t *in, int pitch, int r, int c, int lane,
+                                                 const bf16x8 (&A)[9]) {
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+        const bf16x8 B = *reinterpret_cast<const bf16x8 *>(in + ((r + t / 3) * pitch + c + t % 3) * PSTR +
+                                                           8 * (lane >> 4));
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[t], B, acc, 0, 0, 0);
+    }
+    return acc;
 }
 
 struct TileMap {
@@ -268,38 +308,59 @@ __global__ void __launch_bounds__(256) k_conv32_wgrad(const uint16_t *__restrict
 }
 
 // ---------------------------------------------------------------------------
-// 1 -> 32 layer: a1 = bf16(relu(W1 * bf16(x/scale) + b1)), one thread per pixel.
+// 1 -> 32 layer: a1 = bf16(relu(W1 * bf16(x/scale) + b1)) on MFMA: a block
+// owns an 8 x 32 pixel tile (persistent over tiles), stages bf16(x/scale)
+// with a one-pixel halo in LDS, and each wave computes 4 groups of 16 pixels.
 __global__ void __launch_bounds__(256) k_conv_in(const float *__restrict__ x, float scale,
-                                                 const float *__restrict__ prm, uint16_t *__restrict__ a1, int n,
-                                                 int H, int W) {
+                                                 const float *__restrict__ prm, uint16_t *__restrict__ a1, int H,
+                                                 int W, int ntiles, TileMap tm) {
+    __shared__ float sxs[HS * WS];
     __shared__ float sw[9 * CH + CH];
-    for (int i = threadIdx.x; i < 9 * CH; i += blockDim.x) sw[i] = round_bf16(prm[OFF_W1 + i]);
-    if (threadIdx.x < CH) sw[9 * CH + threadIdx.x] = prm[OFF_B1 + threadIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W1 + i]);
+    if (tid < CH) sw[9 * CH + tid] = prm[OFF_B1 + tid];
     __syncthreads();
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (long)n * H * W) return;
-    const long img = p / ((long)H * W);
-    const int rem = (int)(p - img * H * W), py = rem / W, px = rem - py * W;
-    float xv[9];
+    bf16x8 A[2];
+    conv_in_frag(sw, lane, A);
+    const int j = lane & 15, g = lane >> 4;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();  // the previous tile's reads of sxs are done
+        for (int i = tid; i < HS * WS; i += 256) {
+            const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
+            sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < 4; q++) {
+            const int r = wave * 2 + (q >> 1), cb = (q & 1) * 16;
+            const bf16x8 B = conv_in_b(sxs, WS, r, cb + j, lane);
+            const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B, z, 0, 0, 0);
+            const floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B, z, 0, 0, 0);
+            const int gy = y0 + r, gx = x0 + cb + j;
+            if (gy < H && gx < W) {
+                const long p = ibase + (long)gy * W + gx;
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
-        const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-        xv[t] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? round_bf16(x[(img * H + yy) * W + xx] / scale) : 0.0f;
+                for (int nb = 0; nb < 2; nb++) {
+                    const floatx4 a = nb ? acc1 : acc0;
+                    const int co0 = nb * 16 + 4 * g;
+                    float v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const float zz = a[e] + sw[9 * CH + co0 + e];
+                        v[e] = zz > 0.0f ? zz : 0.0f;
+                    }
+                    uint2 o;
+                    o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                    o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                    *reinterpret_cast<uint2 *>(a1 + p * CH + co0) = o;
+                }
+            }
+        }
     }
-    uint32_t packed[16];
-#pragma unroll
-    for (int co = 0; co < CH; co++) {
-        float z = 0.0f;
-#pragma unroll
-        for (int t = 0; t < 9; t++) z += sw[co * 9 + t] * xv[t];
-        z += sw[9 * CH + co];
-        const uint16_t h = to_bf16(z > 0.0f ? z : 0.0f);
-        if (co & 1) packed[co >> 1] |= (uint32_t)h << 16;
-        else packed[co >> 1] = h;
-    }
-    uint4 *dst = reinterpret_cast<uint4 *>(a1 + p * CH);
-#pragma unroll
-    for (int k = 0; k < 4; k++) dst[k] = make_uint4(packed[4 * k], packed[4 * k + 1], packed[4 * k + 2], packed[4 * k + 3]);
 }
 
 // Weight gradient of the 1 -> 32 layer: partial[block] = (dW1[co][t], db1[co]).
@@ -340,16 +401,18 @@ __global__ void __launch_bounds__(320) k_conv_in_wgrad(const float *__restrict__
 // ---------------------------------------------------------------------------
 // 32 -> 1 layer with the residual: r = W5 * a4 + b5, yn = x/scale + r,
 // y = yn * scale; per-tile sums of r^2 and (x/scale)^2 for the fallback test.
+// One 8 x 32 tile per block: a4 with a one-pixel halo staged in LDS, each wave
+// 4 groups of 16 pixels on MFMA (conv_out_mfma), lanes 0..15 finishing them.
 __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a4, const float *__restrict__ x,
                                                   float scale, const float *__restrict__ prm, float *__restrict__ y,
                                                   float *__restrict__ yn, float *__restrict__ stats, int H, int W,
                                                   TileMap tm) {
     __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
     __shared__ float sw[9 * CH + 1];
-    __shared__ float sred[2][256];
+    __shared__ float sred[2][TH * TW];
     int img, y0, x0;
     tm.of(blockIdx.x, img, y0, x0);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const long ibase = (long)img * H * W;
     for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W5 + i]);
     if (tid == 0) sw[9 * CH] = prm[OFF_B5];
@@ -362,27 +425,29 @@ __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a
             v = *reinterpret_cast<const uint4 *>(a4 + ((ibase + (long)gy * W + gx) * CH + ch * 8));
         *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
     }
+    for (int i = tid; i < TH * TW; i += 256) sred[0][i] = sred[1][i] = 0.0f;
     __syncthreads();
-    const int r = tid / TW, c = tid - r * TW;
-    const int gy = y0 + r, gx = x0 + c;
-    float rr = 0.0f, xx = 0.0f;
-    if (gy < H && gx < W) {
-        float s = 0.0f;
-        for (int t = 0; t < 9; t++)
-            s = dot_tap(sx + ((r + t / 3) * WS + c + t % 3) * PSTR, sw, t, s);
-        s += sw[9 * CH];
-        const long p = ibase + (long)gy * W + gx;
-        const float xn = x[p] / scale;
-        const float v = xn + s;
-        if (yn) yn[p] = v;
-        if (y) y[p] = v * scale;
-        rr = s * s;
-        xx = xn * xn;
+    bf16x8 A[9];
+    conv_out_frag(sw, lane, A);
+    const int j = lane & 15;
+#pragma unroll 1
+    for (int q = 0; q < 4; q++) {
+        const int r = wave * 2 + (q >> 1), c = (q & 1) * 16 + j;
+        const floatx4 acc = conv_out_mfma(sx, WS, r, c, lane, A);
+        const int gy = y0 + r, gx = x0 + c;
+        if (lane < 16 && gy < H && gx < W) {
+            const float s = acc[0] + sw[9 * CH];
+            const long p = ibase + (long)gy * W + gx;
+            const float xn = x[p] / scale;
+            const float v = xn + s;
+            if (yn) yn[p] = v;
+            if (y) y[p] = v * scale;
+            sred[0][r * TW + c] = s * s;
+            sred[1][r * TW + c] = xn * xn;
+        }
     }
-    sred[0][tid] = rr;
-    sred[1][tid] = xx;
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = TH * TW / 2; h > 0; h >>= 1) {
         if (tid < h) {
             sred[0][tid] += sred[0][tid + h];
             sred[1][tid] += sred[1][tid + h];
@@ -500,28 +565,38 @@ __global__ void __launch_bounds__(512) k_cnn_fused(const float *__restrict__ x, 
         sx[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
     }
     __syncthreads();
-    // layer 1 on the a1 region
-    for (int p = tid; p < R1 * C1; p += blockDim.x) {
-        const int r = p / C1, c = p - r * C1, gy = y0 - 4 + r, gx = x0 - 4 + c;
-        const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
-        float xv[9];
+    // layer 1 on the a1 region, 16 pixels per MFMA group (as k_conv_in)
+    {
+        bf16x8 A1[2];
+        conv_in_frag(sw1, lane, A1);
+        const int j = lane & 15, g = lane >> 4, npix = R1 * C1;
+        for (int grp = wave; grp * 16 < npix; grp += nwaves) {
+            const int p = grp * 16 + j, pc = p < npix ? p : npix - 1;
+            const int r = pc / C1, c = pc - r * C1;
+            const bf16x8 B = conv_in_b(sx, XC, r, c, lane);
+            const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[0], B, z, 0, 0, 0);
+            const floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[1], B, z, 0, 0, 0);
+            if (p < npix) {
+                const int gy = y0 - 4 + r, gx = x0 - 4 + c;
+                const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
 #pragma unroll
-        for (int t = 0; t < 9; t++) xv[t] = sx[(r + t / 3) * XC + c + t % 3];
-        uint32_t packedw[16];
+                for (int nb = 0; nb < 2; nb++) {
+                    const floatx4 a = nb ? acc1 : acc0;
+                    const int co0 = nb * 16 + 4 * g;
+                    float v[4];
 #pragma unroll
-        for (int co = 0; co < CH; co++) {
-            float z = 0.0f;
-#pragma unroll
-            for (int t = 0; t < 9; t++) z += sw1[co * 9 + t] * xv[t];
-            z += sw1[9 * CH + co];
-            const uint16_t h = to_bf16(inside && z > 0.0f ? z : 0.0f);
-            if (co & 1) packedw[co >> 1] |= (uint32_t)h << 16;
-            else packedw[co >> 1] = h;
+                    for (int e = 0; e < 4; e++) {
+                        const float zz = a[e] + sw1[9 * CH + co0 + e];
+                        v[e] = inside && zz > 0.0f ? zz : 0.0f;
+                    }
+                    uint2 o;
+                    o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                    o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                    *reinterpret_cast<uint2 *>(bufA + p * PSTR + co0) = o;
+                }
+            }
         }
-        uint4 *dst = reinterpret_cast<uint4 *>(bufA + p * PSTR);
-#pragma unroll
-        for (int k = 0; k < 4; k++)
-            dst[k] = make_uint4(packedw[4 * k], packedw[4 * k + 1], packedw[4 * k + 2], packedw[4 * k + 3]);
     }
     __syncthreads();
     fused_conv32(bufA, C1, bufB, R2, C2, packed, prm + OFF_B2, y0 - 3, x0 - 3, H, W, lane, wave, nwaves);
@@ -532,24 +607,27 @@ __global__ void __launch_bounds__(512) k_cnn_fused(const float *__restrict__ x, 
     fused_conv32(bufA, C2 - 2, bufB, R2 - 4, C2 - 4, packed + 2 * PACK_LAYER, prm + OFF_B4, y0 - 1, x0 - 1, H, W,
                  lane, wave, nwaves);
     __syncthreads();
-    // layer 5 + residual, one output pixel per thread
+    // layer 5 + residual, 16 pixels per MFMA group (as k_conv_out)
     const int C4 = C2 - 4;
-    float rr = 0.0f, xx = 0.0f;
-    for (int q = tid; q < FH * FW; q += blockDim.x) {
-        const int r = q / FW, c = q - r * FW, gy = y0 + r, gx = x0 + c;
-        if (gy < H && gx < W) {
-            float s = 0.0f;
-            for (int t = 0; t < 9; t++) s = dot_tap(bufB + ((r + t / 3) * C4 + c + t % 3) * PSTR, sw5, t, s);
-            s += sw5[9 * CH];
-            const long p = ibase + (long)gy * W + gx;
-            const float xn = x[p] / scale;
-            y[p] = (xn + s) * scale;
-            rr += s * s;
-            xx += xn * xn;
+    for (int i = tid; i < 2 * FH * FW; i += blockDim.x) sred[i] = 0.0f;
+    __syncthreads();
+    {
+        bf16x8 A5[9];
+        conv_out_frag(sw5, lane, A5);
+        const int j = lane & 15;
+        for (int grp = wave; grp * 16 < FH * FW; grp += nwaves) {
+            const int q = grp * 16 + j, r = q / FW, c = q - r * FW, gy = y0 + r, gx = x0 + c;
+            const floatx4 acc = conv_out_mfma(bufB, C4, r, c, lane, A5);
+            if (lane < 16 && gy < H && gx < W) {
+                const float s = acc[0] + sw5[9 * CH];
+                const long p = ibase + (long)gy * W + gx;
+                const float xn = x[p] / scale;
+                y[p] = (xn + s) * scale;
+                sred[q] = s * s;
+                sred[FH * FW + q] = xn * xn;
+            }
         }
     }
-    sred[tid] = rr;
-    sred[FH * FW + tid] = xx;
     __syncthreads();
     for (int h = FH * FW / 2; h > 0; h >>= 1) {
         for (int i = tid; i < h; i += blockDim.x) {
@@ -779,10 +857,10 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     const int ntiles = tm.tiles_x * tm.tiles_y * n;
     const long npx = (long)n * H * W;
     const uint16_t *pk = reinterpret_cast<const uint16_t *>(packed);
-    hipLaunchKernelGGL(k_conv_in, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, x, scale, params, a1, n, H, W);
+    const int nconv = ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS;
+    hipLaunchKernelGGL(k_conv_in, dim3(nconv), dim3(256), 0, s, x, scale, params, a1, H, W, ntiles, tm);
     uint16_t *acts[4] = {a1, a2, a3, a4};
     const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
-    const int nconv = ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS;
     for (int l = 0; l < 3; l++)
         hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
                            params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, ntiles, tm);
