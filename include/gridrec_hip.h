@@ -250,6 +250,14 @@ int gr_plan_destroy(gr_plan *plan);
  * callers that choose when the build's few milliseconds are spent. */
 int gr_plan_build_grid_operator(gr_plan *plan, void *stream);
 
+/* Workgroup width of the plan's column transform: cols columns per workgroup
+ * (1, 2, 4 or 8 side by side while they fit a workgroup and the CU's LDS, or
+ * one after another while cols * M <= 16384); 0 restores the automatic choice.
+ * Results do not depend on it (bitwise).  GR_EINVAL for a width neither form
+ * takes.  No counterpart in the reference (grid.h): a launch-tuning knob, the
+ * one the batch queue's autotuner sets per dataset. */
+int gr_plan_set_cols_per_block(gr_plan *plan, int cols);
+
 /* Per-kernel HIP-event timing of gr_recon_device: an event pair around each
  * launch, resolved (waited for) by gr_get_timing. */
 int gr_set_timing(gr_plan *plan, int enable);

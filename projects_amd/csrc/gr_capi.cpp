@@ -23,6 +23,7 @@
 #include <tuple>
 #include <vector>
 
+#include "gr_fft.hpp"
 #include "gr_plan.h"
 
 namespace {
@@ -272,6 +273,30 @@ std::vector<float2> twiddles(long n, int sign) {
     return t;
 }
 
+// The twiddles of an n-point transform in the pass order gr_lds_fft reads them
+// (gr_fft.hpp): the values of twiddles(n, sign), regrouped per pass, first
+// for passes up to radix 16, then for passes up to radix 4.
+std::vector<float2> pass_twiddles(long n, int sign) {
+    const std::vector<float2> w = twiddles(n, sign);
+    std::vector<float2> out;
+    out.reserve(gr_twp_len(n, 16) + gr_twp_len(n, 4));
+    for (int max_radix : {16, 4}) {
+        long Ns = 1;
+        while (max_radix >= 16 && Ns * 16 <= n) {
+            for (int r = 1; Ns > 1 && r < 16; r++)
+                for (long k = 0; k < Ns; k++) out.push_back(w[r * k * (n / (Ns * 16))]);
+            Ns *= 16;
+        }
+        while (Ns * 4 <= n) {
+            for (int r = 1; Ns > 1 && r < 4; r++)
+                for (long k = 0; k < Ns; k++) out.push_back(w[r * k * (n / (Ns * 4))]);
+            Ns *= 4;
+        }
+        for (long k = 0; Ns < n && k < Ns; k++) out.push_back(w[k]);
+    }
+    return out;
+}
+
 hipEvent_t pool_event(gr_plan *pl) {
     if (!pl->event_pool.empty()) {
         hipEvent_t e = pl->event_pool.back();
@@ -449,9 +474,10 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         }
     }
     // twiddles (shared per size)
-    CK(shared_table(params->device, kTwPlus, g.pdim, [&] { return twiddles(g.pdim, +1); }, &pl->d_tw_row, nullptr));
-    CK(shared_table(params->device, kTwMinus, g.M, [&] { return twiddles(g.M, -1); }, &pl->d_tw_grid, nullptr));
-    pl->dev_bytes += sizeof(float2) * (g.pdim + g.M);
+    CK(shared_table(params->device, kTwPlus, g.pdim, [&] { return pass_twiddles(g.pdim, +1); }, &pl->d_tw_row, nullptr));
+    CK(shared_table(params->device, kTwMinus, g.M, [&] { return pass_twiddles(g.M, -1); }, &pl->d_tw_grid, nullptr));
+    pl->dev_bytes += sizeof(float2) * (gr_twp_len(g.pdim, 16) + gr_twp_len(g.pdim, 4) + gr_twp_len(g.M, 16) +
+                                       gr_twp_len(g.M, 4));
     // angle keys: direction of each sample line, atan2(SINE, COSE) mod pi, sorted
     {
         std::vector<double> key(n_ang);
@@ -862,6 +888,13 @@ int gr_plan_build_grid_operator(gr_plan *pl, void *stream) {
     GR_HIP(hipSetDevice(pl->prm.device));
     if (pl->gop_state != 0) return GR_OK;
     return build_gop(pl, (hipStream_t)stream, true, SIZE_MAX);
+}
+
+int gr_plan_set_cols_per_block(gr_plan *pl, int cols) {
+    if (!pl) return fail(GR_EINVAL, "null plan");
+    if (cols < 0 || (cols > 0 && !gr_cols_width_ok(pl->g.M, cols))) return fail(GR_EINVAL, "unsupported column width");
+    pl->cols_per_block = cols;
+    return GR_OK;
 }
 
 int gr_plan_destroy(gr_plan *pl) {

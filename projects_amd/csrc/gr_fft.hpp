@@ -12,7 +12,13 @@
 // the reference's four1 with isign=+1 and fourn with isign=-1,
 // gridrec/GridrecCPP/Gridrec5.2/fftgr.cpp:8-120).
 //
-// tw[m] = e^{SIGN 2 pi i m / N}, m < N, rounded from double on the host.
+// Twiddles come pass-ordered (gr_twp_len, built on the host from the table
+// w[m] = e^{SIGN 2 pi i m / N} rounded from double): for each pass with
+// Ns > 1, in pass order, the entries [r-1][k] = w[r k N/(Ns R)] for
+// r = 1 .. R-1 and k < Ns (R the pass's radix; the radix-2 pass: [k] = w[k]),
+// first the passes of max_radix 16, then those of max_radix 4.  A pass's
+// lanes then read consecutive entries -- one or a few cache lines per load
+// instead of up to one per lane -- and multiply by the very same values.
 // MAXB bounds the radix-4 butterflies per thread: N/4 <= MAXB * nthreads.
 //
 // LDS layout: element i of the row lives at gr_lds_idx(i) = i XOR ((i >> 4)
@@ -28,6 +34,22 @@
 #include <hip/hip_runtime.h>
 
 __host__ __device__ __forceinline__ int gr_lds_idx(int i) { return i ^ ((i >> 4) & 31); }
+
+// Entries of the pass-ordered twiddles of an N-point transform whose passes
+// go up to max_radix (16 or 4).
+__host__ __device__ __forceinline__ long gr_twp_len(long N, int max_radix) {
+    long Ns = 1, len = 0;
+    while (max_radix >= 16 && Ns * 16 <= N) {
+        if (Ns > 1) len += 15 * Ns;
+        Ns *= 16;
+    }
+    while (Ns * 4 <= N) {
+        if (Ns > 1) len += 3 * Ns;
+        Ns *= 4;
+    }
+    if (Ns < N) len += Ns;
+    return len;
+}
 __host__ __device__ __forceinline__ long gr_lds_len(long N) { return N; }
 
 __device__ __forceinline__ float2 gr_cmul(float2 a, float2 b) {
@@ -75,11 +97,11 @@ template <int SIGN, int MAXB>
 __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr,
                                            int max_radix = 16) {
     int Ns = 1;
+    const float2 *__restrict__ twp = tw + (max_radix >= 16 ? 0 : gr_twp_len(N, 16));
     constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
     const int nb16 = N >> 4;
     while (max_radix >= 16 && Ns * 16 <= N) {
         float2 v[MAXB16][16];
-        const int stride = N / (Ns * 16);
 #pragma unroll
         for (int c = 0; c < MAXB16; c++) {
             const int j = tid + c * nthr;
@@ -88,9 +110,8 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
 #pragma unroll
                 for (int r = 0; r < 16; r++) v[c][r] = buf[gr_lds_idx(j + r * nb16)];
                 if (Ns > 1) {
-                    const int m = k * stride;
 #pragma unroll
-                    for (int r = 1; r < 16; r++) v[c][r] = gr_cmul(v[c][r], tw[r * m]);
+                    for (int r = 1; r < 16; r++) v[c][r] = gr_cmul(v[c][r], twp[(r - 1) * Ns + k]);
                 }
                 gr_dft16<SIGN>(v[c]);
             }
@@ -107,12 +128,12 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             }
         }
         __syncthreads();
+        if (Ns > 1) twp += 15 * Ns;
         Ns <<= 4;
     }
     const int nb4 = N >> 2;
     while (Ns * 4 <= N) {
         float2 v[MAXB][4];
-        const int stride = N / (Ns * 4);
 #pragma unroll
         for (int c = 0; c < MAXB; c++) {
             const int j = tid + c * nthr;
@@ -121,10 +142,9 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[c][r] = buf[gr_lds_idx(j + r * nb4)];
                 if (Ns > 1) {
-                    const int m = k * stride;
-                    v[c][1] = gr_cmul(v[c][1], tw[m]);
-                    v[c][2] = gr_cmul(v[c][2], tw[2 * m]);
-                    v[c][3] = gr_cmul(v[c][3], tw[3 * m]);
+                    v[c][1] = gr_cmul(v[c][1], twp[k]);
+                    v[c][2] = gr_cmul(v[c][2], twp[Ns + k]);
+                    v[c][3] = gr_cmul(v[c][3], twp[2 * Ns + k]);
                 }
                 gr_dft4<SIGN>(v[c][0], v[c][1], v[c][2], v[c][3]);
             }
@@ -141,6 +161,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             }
         }
         __syncthreads();
+        if (Ns > 1) twp += 3 * Ns;
         Ns <<= 2;
     }
     if (Ns < N) {  // N = 2 Ns: one radix-2 pass, twiddle e^{SIGN 2 pi i k/N}
@@ -152,7 +173,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
             if (j < nb2) {
                 const int k = j & (Ns - 1);
                 const float2 x0 = buf[gr_lds_idx(j)];
-                const float2 x1 = gr_cmul(buf[gr_lds_idx(j + nb2)], tw[k]);
+                const float2 x1 = gr_cmul(buf[gr_lds_idx(j + nb2)], twp[k]);
                 a[c] = make_float2(x0.x + x1.x, x0.y + x1.y);
                 b[c] = make_float2(x0.x - x1.x, x0.y - x1.y);
             }

@@ -44,8 +44,8 @@ struct GrDevTables {
     const float *wtbl;             // ltbl+1
     const float *dwtbl = nullptr;  // ltbl+1 backward differences: the INTERP convolvent (null: nearest)
     const float *winv;             // M0
-    const float2 *tw_row;          // pdim: e^{+2 pi i m/pdim}
-    const float2 *tw_grid;         // M:    e^{-2 pi i m/M}
+    const float2 *tw_row;          // pdim: e^{+2 pi i m/pdim}, pass-ordered (gr_twp_len)
+    const float2 *tw_grid;         // M:    e^{-2 pi i m/M}, pass-ordered
     const float *akey;             // n_ang angle keys in [0, pi), sorted
     const int *aord;               // n_ang angle index of each sorted key
     const float4 *ang;             // n_ang in sorted order: COSE, SINE, 1/COSE, 1/SINE (0 if |.| <= 1e-6)
@@ -128,6 +128,9 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
                                long out_slice_stride, long out_row_stride, int n_slices, int pair0,
                                int pairs, int cols_per_block, hipStream_t s);
+// Whether gr_launch_ifft_cols takes cols columns per workgroup for M (else it
+// falls back to its automatic width).
+bool gr_cols_width_ok(long M, int cols);
 
 // FP64 tables on the device, rounded to float: filphase (half), wtbl (ltbl+1),
 // winv (M0).

@@ -66,35 +66,44 @@ __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
     return (long)(x % 8u) * (ngroups / 8u) + x / 8u;
 }
 
-// ONE: one column per workgroup (the default), known at compile time -- the
-// column loop then folds away and the kernel needs 62 VGPRs instead of 112,
-// so LDS, not registers, sets the workgroups per CU.
-template <int MAXB, bool ONE>
-__global__ void __launch_bounds__(512) k_ifft_cols(ColsArgs A) {
+// CW > 0: CW neighbouring columns per workgroup, transformed side by side by
+// CW groups of nthr/CW threads (the workgroup's barriers are shared, so the
+// groups run the same passes in lockstep); the fill reads the CW columns of a
+// row from one cache line.  The column gathers -- one line per row and column
+// group, strided by a row of Z -- are what bounds this kernel, so wider groups
+// fetch fewer lines for the same bytes.  CW = 0: A.cw columns one after
+// another on all threads (small M, where CW * M is small).
+template <int MAXB, int CW>
+__global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
     extern __shared__ float2 cols[];  // cw columns of gr_lds_len(M) slots
     const int p = blockIdx.y;
-    const int M = A.M, cw = ONE ? 1 : A.cw;
+    const int M = A.M, cw = CW ? CW : A.cw;
     const long c0 = column_group(blockIdx.x, gridDim.x) * cw;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nthr = CW ? (int)blockDim.x / CW : (int)blockDim.x;
+    const int tid = CW ? (int)threadIdx.x % nthr : (int)threadIdx.x;
+    const int grp = CW ? (int)threadIdx.x / nthr : 0;
     const float2 *zp = A.z + (long)p * M * A.out_n;
     const int out_n = (int)A.out_n;
     if (cw == 1) {
         const float2 *zc = zp + c0;
         for (int b = tid; b < M; b += nthr) cols[gr_lds_idx(b)] = c0 < out_n ? zc[(long)b * out_n] : make_float2(0.0f, 0.0f);
     } else {
-        for (int idx = tid; idx < cw * M; idx += nthr) {  // cw * M <= 16384
+        for (int idx = threadIdx.x; idx < cw * M; idx += blockDim.x) {  // CW: cw * M <= 20480; else <= 16384
             const int b = idx / cw, c = idx - b * cw;
             cols[c * gr_lds_len(M) + gr_lds_idx(b)] =
                 (c0 + c < out_n) ? zp[(long)b * out_n + c0 + c] : make_float2(0.0f, 0.0f);
         }
     }
     __syncthreads();
-    for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
+    if (CW)
+        gr_lds_fft<-1, MAXB>(cols + grp * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
+    else
+        for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
 
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     float *o1 = A.out + s1 * A.out_slice_stride;
     float *o2 = s2 < A.n_slices ? A.out + s2 * A.out_slice_stride : nullptr;
-    for (int c = 0; c < cw; c++) {
+    for (int c = CW ? grp : 0; c < (CW ? grp + 1 : cw); c++) {
         const long jj = c0 + c;
         if (jj >= A.out_n) break;
         const float cu = A.winv[A.out_off + jj];
@@ -133,17 +142,27 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
     return hipGetLastError();
 }
 
-// Columns per workgroup by default: one.  Wider groups read Z in longer
-// contiguous runs but hold cw x M x 8 bytes of LDS, which caps the
-// workgroups per CU; with the column groups numbered per XCD (neighbouring
-// columns share an L2) one column per workgroup measured fastest for M = 1024
-// .. 8192 on MI355X, as the queue's autotuner also finds.  Callers may ask for
-// a width (gr_plan.cols_per_block).
+// Columns per workgroup.  Side by side (CW > 0) while CW FFT thread groups
+// fit a workgroup (<= 1024 threads) and CW columns fit the CU's LDS (160 KiB);
+// beyond that, one after another on all threads while cw * M <= 16384.  The
+// default is the widest of 4, 2, 1 side by side that fits: the column gathers
+// bound the kernel and a line then serves that many columns (at M = 4096 on
+// MI355X: 0.57 ms per launch of 8 pairs with 1, 0.50 with 2, 0.48 with 4).
+// Callers may ask for a width (gr_plan.cols_per_block).
+
+static bool cols_side_by_side(long M, int cw) {
+    return cw >= 1 && cw <= 8 && (cw & (cw - 1)) == 0 && (long)cw * gr_fft_threads(M) <= 1024 &&
+           (long)cw * M * 8 <= 163840;
+}
+
+bool gr_cols_width_ok(long M, int cols) {
+    return cols_side_by_side(M, cols) || (cols >= 1 && (long)cols * M <= 16384);
+}
+
 static int cols_per_block(long M, long out_n, int pairs) {
-    (void)M;
     (void)out_n;
     (void)pairs;
-    return 1;
+    return cols_side_by_side(M, 4) ? 4 : cols_side_by_side(M, 2) ? 2 : 1;
 }
 
 hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const float2 *z, float *out,
@@ -157,7 +176,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_slice_stride = out_slice_stride;
     A.out_row_stride = out_row_stride;
     A.M = (int)g.M;
-    A.cw = cols > 0 && (long)cols * g.M <= 16384 ? cols : cols_per_block(g.M, g.out_n, pairs);
+    A.cw = cols > 0 && gr_cols_width_ok(g.M, cols) ? cols : cols_per_block(g.M, g.out_n, pairs);
     A.n_slices = n_slices;
     A.pair0 = pair0;
     A.radix = g.fft_radix;
@@ -165,19 +184,23 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_off = g.out_off;
     A.out_n = g.out_n;
     const int nthr = gr_fft_threads(g.M);
-    const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(nthr);
+    const bool side = cols_side_by_side(g.M, A.cw);
+    const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(side ? A.cw * nthr : nthr);
     const size_t lds = sizeof(float2) * A.cw * gr_lds_len(g.M);
+#define GR_COLS_K(MB, CW)                                                                \
+    do {                                                                                 \
+        hipError_t e_ = gr_allow_lds(k_ifft_cols<MB, CW>, lds);                          \
+        if (e_ != hipSuccess) return e_;                                                 \
+        hipLaunchKernelGGL((k_ifft_cols<MB, CW>), grid, block, lds, s, A);               \
+    } while (0)
 #define GR_COLS(MB)                                                                      \
     do {                                                                                 \
-        if (A.cw == 1) {                                                                 \
-            hipError_t e_ = gr_allow_lds(k_ifft_cols<MB, true>, lds);                    \
-            if (e_ != hipSuccess) return e_;                                             \
-            hipLaunchKernelGGL((k_ifft_cols<MB, true>), grid, block, lds, s, A);         \
-        } else {                                                                         \
-            hipError_t e_ = gr_allow_lds(k_ifft_cols<MB, false>, lds);                   \
-            if (e_ != hipSuccess) return e_;                                             \
-            hipLaunchKernelGGL((k_ifft_cols<MB, false>), grid, block, lds, s, A);        \
-        }                                                                                \
+        if (!side) GR_COLS_K(MB, 0);                                                     \
+        else if (A.cw == 1) GR_COLS_K(MB, 1);                                            \
+        else if (A.cw == 2) GR_COLS_K(MB, 2);                                            \
+        else if (A.cw == 4) GR_COLS_K(MB, 4);                                            \
+        else if (A.cw == 8) GR_COLS_K(MB, 8);                                            \
+        else GR_COLS_K(MB, 0);                                                           \
     } while (0)
     switch (maxb_of(g.M, nthr)) {
     case 0:
@@ -188,6 +211,7 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     case 16: { GR_COLS(16); } break;
     default: return hipErrorInvalidValue;
     }
+#undef GR_COLS_K
 #undef GR_COLS
     return hipGetLastError();
 }

@@ -103,6 +103,7 @@ _SIGS = {
                                   C.c_void_p]),
     "gr_plan_destroy": (C.c_int, [C.c_void_p]),
     "gr_plan_build_grid_operator": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "gr_plan_set_cols_per_block": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "gr_get_timing": (C.c_int, [C.c_void_p, C.POINTER(GrTiming), C.c_int]),
     "gr_phantom_sinograms": (C.c_int, [C.c_void_p, C.c_long, C.c_int, C.c_long, C.c_ulonglong, C.c_int, C.c_int,
@@ -228,6 +229,11 @@ class Plan:
         inf = GrInfo()
         _check(lib().gr_plan_info(self._h, C.byref(inf)), "gr_plan_info")
         return {f: getattr(inf, f) for f, _ in GrInfo._fields_}
+
+    def set_cols_per_block(self, cols: int):
+        """Workgroup width of the column transform (gr_plan_set_cols_per_block;
+        0 = automatic).  Images do not depend on it."""
+        _check(lib().gr_plan_set_cols_per_block(self._h, int(cols)), "gr_plan_set_cols_per_block")
 
     def build_operator(self, stream=None):
         """Build the gather operator now (gr_plan_build_grid_operator)."""

@@ -182,6 +182,40 @@ def test_pair_grouping_is_bitwise_neutral(accum, grid_op):
         assert torch.equal(out, ref), mp
 
 
+def test_column_widths_are_bitwise_neutral():
+    """The column transform's workgroup width (columns side by side, or one
+    after another) changes only the launch shape: every width a plan accepts
+    gives bitwise the images of one column per workgroup, from M = 128 to
+    M = 8192; widths neither form takes are refused."""
+    cases = [dict(n_ang=40, n_det=96, center=47.25), dict(n_ang=60, n_det=300, center=150.5),
+             dict(n_ang=50, n_det=700, center=349.0), dict(n_ang=40, n_det=1500, center=751.0),
+             dict(n_ang=45, n_det=2048, center=1024.3, pad="half"),
+             dict(n_ang=30, n_det=2560, center=1280.0, pad="half")]
+    rng = np.random.default_rng(33)
+    seen = set()
+    for case in cases:
+        sino = torch.from_numpy(rng.standard_normal((3, case["n_ang"], case["n_det"])).astype(np.float32)).cuda()
+        p = gr.Plan(**case)
+        M = p.info["M"]
+        seen.add(M)
+        p.set_cols_per_block(1)
+        ref = p.recon(sino)
+        torch.cuda.synchronize()
+        assert torch.isfinite(ref).all()
+        for cols in (0, 2, 4, 8, 16):
+            try:
+                p.set_cols_per_block(cols)
+            except gr.GridrecError:
+                print(f"M={M}: width {cols} refused", flush=True)
+                assert cols * M > 16384, (M, cols)
+                continue
+            out = p.recon(sino)
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (M, cols)
+        p.close()
+    assert {4096, 8192} <= seen, seen
+
+
 def test_recon_host_equals_device_and_is_deterministic():
     g = O.Geometry(n_ang=90, n_det=128, center=63.5)
     rng = np.random.default_rng(9)
