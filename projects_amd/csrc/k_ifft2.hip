@@ -32,19 +32,69 @@ struct RowsArgs {
     float rmax;  // tiles beyond it were not written by the gridding step: zeros
 };
 
+// The tiles of tile row ty that the gridding step wrote (gr_tile_active):
+// [lo, hi], empty when lo > hi.  The predicate is monotone in |cu| (cu is
+// exact; the rounded square, sum and square root are monotone), so these
+// tiles form one interval around the column nearest the centre; a first
+// guess from the real-valued circle is moved to the interval's exact ends by
+// the predicate itself.  Wave-uniform.
+__device__ __forceinline__ void active_tile_range(int ty, int ntx, float M2, float rmax, int &lo, int &hi) {
+    const float c = (M2 - 3.5f) / 8.0f;  // |cu| is smallest at tile column c
+    const int t0 = min(max((int)floorf(c), 0), ntx - 1), t1 = min(t0 + 1, ntx - 1);
+    const int tc = gr_tile_active(t0, ty, M2, rmax) ? t0 : (gr_tile_active(t1, ty, M2, rmax) ? t1 : -1);
+    if (tc < 0) {
+        lo = 1;
+        hi = 0;
+        return;
+    }
+    const float cv = (float)(ty * 8) + 3.5f - M2;
+    const float w = sqrtf(fmaxf(rmax * rmax - cv * cv, 0.0f)) / 8.0f;
+    lo = max(0, min(tc, (int)floorf(c - w)));
+    hi = min(ntx - 1, max(tc, (int)ceilf(c + w)));
+    while (lo < tc && !gr_tile_active(lo, ty, M2, rmax)) lo++;
+    while (lo > 0 && gr_tile_active(lo - 1, ty, M2, rmax)) lo--;
+    while (hi > tc && !gr_tile_active(hi, ty, M2, rmax)) hi--;
+    while (hi < ntx - 1 && gr_tile_active(hi + 1, ty, M2, rmax)) hi++;
+}
+
+// Loads and LDS stores in loops unrolled to the per-thread bound (M <= 4 MAXB
+// nthr), every load issued before the first store: one memory latency per
+// row instead of one per element.
 template <int MAXB>
 __global__ void __launch_bounds__(512) k_ifft_rows(RowsArgs A) {
     extern __shared__ float2 row[];
+    constexpr int EPT = 4 * MAXB;
     const int b = blockIdx.x, p = blockIdx.y;
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const float2 *src = A.ht + ((long)p * A.M + b) * A.M;
-    const float M2 = (float)(A.M >> 1);
-    for (int i = tid; i < A.M; i += nthr)
-        row[gr_lds_idx(i)] = gr_tile_active(i >> 3, b >> 3, M2, A.rmax) ? src[i] : make_float2(0.0f, 0.0f);
+    const int M = A.M;
+    const float2 *src = A.ht + ((long)p * M + b) * M;
+    int lo, hi;
+    active_tile_range(b >> 3, M >> 3, (float)(M >> 1), A.rmax, lo, hi);
+    float2 v[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int i = tid + r * nthr, tx = i >> 3;
+        v[r] = i < M && tx >= lo && tx <= hi ? src[i] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int i = tid + r * nthr;
+        if (i < M) row[gr_lds_idx(i)] = v[r];
+    }
     __syncthreads();
-    gr_lds_fft<-1, MAXB>(row, A.M, A.tw, tid, nthr, A.radix);
-    float2 *dst = A.z + ((long)p * A.M + b) * A.out_n;
-    for (long jj = tid; jj < A.out_n; jj += nthr) dst[jj] = row[gr_lds_idx((int)corner_index(A.out_off + jj, A.M, A.M02))];
+    gr_lds_fft<-1, MAXB>(row, M, A.tw, tid, nthr, A.radix);
+    float2 *dst = A.z + ((long)p * M + b) * A.out_n;
+    const int out_n = (int)A.out_n, off = (int)A.out_off, m02 = (int)A.M02;
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int jj = tid + r * nthr;
+        if (jj < out_n) v[r] = row[gr_lds_idx((int)corner_index(off + jj, M, m02))];
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int jj = tid + r * nthr;
+        if (jj < out_n) dst[jj] = v[r];
+    }
 }
 
 struct ColsArgs {
@@ -84,7 +134,24 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
     const int grp = CW ? (int)threadIdx.x / nthr : 0;
     const float2 *zp = A.z + (long)p * M * A.out_n;
     const int out_n = (int)A.out_n;
-    if (cw == 1) {
+    constexpr int EPT = 4 * MAXB;  // rows per thread and column: M <= 4 MAXB nthr
+    if (CW) {
+        // unrolled, every load before the first LDS store; the CW columns of
+        // a row come from neighbouring lanes
+        float2 v[EPT];
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int idx = (int)threadIdx.x + r * (int)blockDim.x;
+            const int b = idx / CW, c = idx - b * CW;
+            v[r] = b < M && c0 + c < out_n ? zp[(long)b * out_n + c0 + c] : make_float2(0.0f, 0.0f);
+        }
+#pragma unroll
+        for (int r = 0; r < EPT; r++) {
+            const int idx = (int)threadIdx.x + r * (int)blockDim.x;
+            const int b = idx / CW, c = idx - b * CW;
+            if (b < M) cols[c * gr_lds_len(M) + gr_lds_idx(b)] = v[r];
+        }
+    } else if (cw == 1) {
         const float2 *zc = zp + c0;
         for (int b = tid; b < M; b += nthr) cols[gr_lds_idx(b)] = c0 < out_n ? zc[(long)b * out_n] : make_float2(0.0f, 0.0f);
     } else {
@@ -110,6 +177,28 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
         const float2 *col = cols + c * gr_lds_len(M);
         float *r1 = o1 + jj * A.out_row_stride, *r2 = o2 ? o2 + jj * A.out_row_stride : nullptr;
         const int off = (int)A.out_off, m02 = (int)A.M02;
+        if (CW) {  // unrolled: the winv loads of all of a thread's outputs in flight at once
+            float wk[EPT];
+            float2 v[EPT];
+#pragma unroll
+            for (int r = 0; r < EPT; r++) {
+                const int kk = tid + r * nthr, k = off + kk;
+                if (kk < out_n) {
+                    wk[r] = A.winv[k];
+                    v[r] = col[gr_lds_idx(k < m02 ? M - m02 + k : k - m02)];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < EPT; r++) {
+                const int kk = tid + r * nthr;
+                if (kk < out_n) {
+                    const float corrn = cu * wk[r];  // grid.cpp:401
+                    r1[kk] = corrn * v[r].x;
+                    if (r2) r2[kk] = corrn * v[r].y;
+                }
+            }
+            continue;
+        }
         for (int kk = tid; kk < out_n; kk += nthr) {
             const int k = off + kk;
             const float corrn = cu * A.winv[k];  // grid.cpp:401

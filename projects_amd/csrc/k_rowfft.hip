@@ -86,9 +86,16 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     const bool has2 = s2 < A.n_slices;
 
-    for (int j = tid; j < A.pdim; j += nthr) {
+    // loops unrolled to the per-thread bound (pdim <= 4 MAXB nthr), every
+    // load issued before the first LDS store: one memory latency per row
+    // instead of one per element
+    constexpr int EPT = 4 * MAXB;
+    float2 v[EPT];
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int j = tid + r * nthr;
         float re = 0.0f, im = 0.0f;
-        if (j < A.n_eff) {
+        if (j < A.pdim && j < A.n_eff) {
             int q = j;
             if (A.pad) {
                 q = j - A.pad_off;
@@ -114,7 +121,12 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
                                       P.white1 + f2, P.white2 + f2, A.n_in, P.xoffset, q);
             }
         }
-        row[gr_lds_idx(j)] = make_float2(re, im);
+        v[r] = make_float2(re, im);
+    }
+#pragma unroll
+    for (int r = 0; r < EPT; r++) {
+        const int j = tid + r * nthr;
+        if (j < A.pdim) row[gr_lds_idx(j)] = v[r];
     }
     __syncthreads();
     gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr, A.radix);
@@ -123,14 +135,24 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     float2 *o1 = A.spec1 + rec;
     float2 *o2 = A.spec2 + rec;
     const float offset = A.roi_offset[n];
-    for (int j = tid; j < A.half; j += nthr) {
+    constexpr int EPH = 2 * MAXB;  // half = pdim/2 <= 2 MAXB nthr
+    float2 fp[EPH];
+#pragma unroll
+    for (int r = 0; r < EPH; r++) {
+        const int j = tid + r * nthr;
+        if (j < A.half) fp[r] = A.filphase[j];
+    }
+#pragma unroll
+    for (int r = 0; r < EPH; r++) {
+        const int j = tid + r * nthr;
+        if (j >= A.half) continue;
         if (j == 0) {
             o1[0] = make_float2(0.0f, 0.0f);
             o2[0] = make_float2(0.0f, 0.0f);
             continue;
         }
         const long o = (long)j * kSpecGroup;
-        float2 ct = A.filphase[j];
+        float2 ct = fp[r];
         if (A.flag) {  // grid.cpp:267-269
             const float arg = (float)j * offset;
             const float phr = cosf(arg), phi = -sinf(arg);
