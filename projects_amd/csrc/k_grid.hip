@@ -455,18 +455,27 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
     const unsigned n = O.tile_n[ti] & kGopRows;
     float4 acc[1] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
     float2 part[1] = {make_float2(0.0f, 0.0f)};
+    auto entry = [&](unsigned k) {
+        const uint2 t = gop_load(e + 64ul * min(k, n - 1));
+        return k < n ? t : make_uint2(0u, 0u);
+    };
+    // software-pipelined: the next batch's entries are in flight while this
+    // batch's samples are gathered and added, so a batch costs one memory
+    // round trip instead of two
+    uint2 t[kHeavyBatch];
+#pragma unroll
+    for (int i = 0; i < kHeavyBatch; i++) t[i] = entry(i);
     for (unsigned k0 = 0; k0 < n; k0 += kHeavyBatch) {
-        uint2 t[kHeavyBatch];
         float2 v[kHeavyBatch][1];
 #pragma unroll
-        for (int i = 0; i < kHeavyBatch; i++) {
-            t[i] = gop_load(e + 64ul * min(k0 + i, n - 1));
-            if (k0 + i >= n) t[i] = make_uint2(0u, 0u);
-        }
-#pragma unroll
         for (int i = 0; i < kHeavyBatch; i++) gop_gather<1>(t[i], base1, base2, v[i]);
+        uint2 tn[kHeavyBatch];
+#pragma unroll
+        for (int i = 0; i < kHeavyBatch; i++) tn[i] = entry(k0 + kHeavyBatch + i);
 #pragma unroll
         for (int i = 0; i < kHeavyBatch; i++) gop_consume<ACCUM, 1>(t[i], v[i], acc, part);
+#pragma unroll
+        for (int i = 0; i < kHeavyBatch; i++) t[i] = tn[i];
     }
     store_cells<ACCUM, 1>(A, pair, 1, a, b, acc);
 }
