@@ -61,8 +61,9 @@ def parse():
                     help="all: the plain headline line with the batched queue, the precision study and the "
                          "learned sparse-angle path after it")
     ap.add_argument("--learned-pairs", type=int, default=20000, help="training pairs (512^2, 90 / 900 angles)")
-    ap.add_argument("--learned-seconds", type=float, default=90.0, help="training time budget")
-    ap.add_argument("--learned-steps", type=int, default=0, help="training step cap (0 = time budget only)")
+    ap.add_argument("--learned-seconds", type=float, default=150.0, help="training time cap")
+    ap.add_argument("--learned-steps", type=int, default=8000,
+                    help="training steps (0 = time budget only); a fixed count makes the trained network reproducible")
     ap.add_argument("--learned-slices", type=int, default=256, help="inference slices (2048^2, 150 angles)")
     ap.add_argument("--learned-truth-slices", type=int, default=4,
                     help="inference slices also compared with the analytic phantom")

@@ -234,10 +234,15 @@ def make_pairs(n: int, size: int = 512, sparse: int = 90, dense: int = 900, seed
 
 
 def train(net: Corrector, inp, tgt, steps: int = 0, seconds: float = 0.0, batch: int = 32, lr: float = 1e-3,
-          seed: int = 0, log_every: int = 50, progress=None):
+          seed: int = 0, log_every: int = 50, progress=None, lr_final: float = 0.05):
     """Adam steps on random batches of (inp, tgt); stops after `steps` steps or
-    `seconds` of wall time, whichever is set (both: the first reached).
+    `seconds` of wall time, whichever is set (both: the first reached).  The
+    learning rate follows a cosine from lr down to lr * lr_final over the
+    budget (the larger of the step and time fractions used), so the weights
+    the run ends with are a settled iterate, not one of the noisy ones a
+    constant rate leaves (lr_final = 1: constant).
     Returns the loss history [(step, loss)] sampled every log_every steps."""
+    import math
     import time
 
     import torch
@@ -251,7 +256,9 @@ def train(net: Corrector, inp, tgt, steps: int = 0, seconds: float = 0.0, batch:
     while True:
         idx = torch.randint(0, n, (batch,), device="cuda", generator=g)
         xb, tb = inp.index_select(0, idx).contiguous(), tgt.index_select(0, idx).contiguous()
-        loss = net.train_step(xb, tb, lr=lr)
+        f = max(k / steps if steps else 0.0, (time.perf_counter() - t0) / seconds if seconds else 0.0)
+        rate = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * min(f, 1.0))))
+        loss = net.train_step(xb, tb, lr=rate)
         k += 1
         if k % log_every == 0 or k == 1:
             hist.append((k, float(loss.cpu())))
