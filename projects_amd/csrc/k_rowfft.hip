@@ -71,18 +71,27 @@ __device__ __forceinline__ float shifted(const unsigned short *raw, const float 
     return shift_interp([&](int q) { return normalised<PRE>(raw, dark, w1, w2, q); }, n, xoffset, k);
 }
 
-// Workgroup b takes angle (b / 8 / pairs) * 8 + b % 8 of pair (b / 8) % pairs:
-// the pairs of one angle run back to back on one XCD (workgroup b runs on XCD
-// b mod 8), where their 8-byte stores into the interleaved spectra fill whole
-// lines of that XCD's L2.
-template <int MAXB, int PRE>
-__global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
-    extern __shared__ float2 row[];
-    const int r = blockIdx.x >> 3;
-    const int p = r % A.pairs;
-    const int n = (r / A.pairs) * 8 + (blockIdx.x & 7);
+// A workgroup transforms CW pairs of one angle side by side (CW groups of
+// nthr = blockDim / CW threads, each with its own LDS row; the workgroup's
+// barriers are shared, so the groups run the same passes in lockstep) and
+// writes the CW neighbouring slots of each interleaved record with one store.
+// Workgroup b takes angle (b / 8 / G) * 8 + b % 8 and pair group (b / 8) % G,
+// G = ceil(pairs / CW): the pair groups of one angle run back to back on one
+// XCD (workgroup b runs on XCD b mod 8), where their stores into the
+// interleaved spectra meet in that XCD's L2.
+template <int MAXB, int PRE, int CW>
+__global__ void __launch_bounds__(1024) k_rowfft(RowFftArgs A) {
+    extern __shared__ float2 lds[];
+    const int G = (A.pairs + CW - 1) / CW;
+    const int rb = blockIdx.x >> 3;
+    const int pg = rb % G;
+    const int n = (rb / G) * 8 + (blockIdx.x & 7);
     if (n >= A.n_ang) return;
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nthr = (int)blockDim.x / CW;
+    const int grp = (int)threadIdx.x / nthr, tid = (int)threadIdx.x % nthr;
+    const int p = pg * CW + grp;  // this group's pair
+    const bool valid = p < A.pairs;
+    float2 *row = lds + (long)grp * A.pdim;
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     const bool has2 = s2 < A.n_slices;
 
@@ -95,7 +104,7 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     for (int r = 0; r < EPT; r++) {
         const int j = tid + r * nthr;
         float re = 0.0f, im = 0.0f;
-        if (j < A.pdim && j < A.n_eff) {
+        if (valid && j < A.pdim && j < A.n_eff) {
             int q = j;
             if (A.pad) {
                 q = j - A.pad_off;
@@ -131,41 +140,72 @@ __global__ void __launch_bounds__(512) k_rowfft(RowFftArgs A) {
     __syncthreads();
     gr_lds_fft<+1, MAXB>(row, A.pdim, A.tw, tid, nthr, A.radix);
 
-    const long rec = ((long)(p / kSpecGroup) * A.n_ang + n) * A.half * kSpecGroup + p % kSpecGroup;
+    // epilogue: every thread of the workgroup takes frequencies j and writes
+    // the CW pairs' slots of record j (CW * 8 contiguous bytes; slots of pairs
+    // past the launch's last, whose rows were zeros, get zeros)
+    const int p0 = pg * CW;
+    const long rec = ((long)(p0 / kSpecGroup) * A.n_ang + n) * A.half * kSpecGroup + p0 % kSpecGroup;
     float2 *o1 = A.spec1 + rec;
     float2 *o2 = A.spec2 + rec;
     const float offset = A.roi_offset[n];
-    constexpr int EPH = 2 * MAXB;  // half = pdim/2 <= 2 MAXB nthr
+    const int nall = (int)blockDim.x;
+    constexpr int EPH = (2 * MAXB + CW - 1) / CW;  // half = pdim/2 <= 2 MAXB nthr
     float2 fp[EPH];
 #pragma unroll
     for (int r = 0; r < EPH; r++) {
-        const int j = tid + r * nthr;
+        const int j = (int)threadIdx.x + r * nall;
         if (j < A.half) fp[r] = A.filphase[j];
     }
 #pragma unroll
     for (int r = 0; r < EPH; r++) {
-        const int j = tid + r * nthr;
+        const int j = (int)threadIdx.x + r * nall;
         if (j >= A.half) continue;
+        float2 c1[CW], c2[CW];
         if (j == 0) {
-            o1[0] = make_float2(0.0f, 0.0f);
-            o2[0] = make_float2(0.0f, 0.0f);
-            continue;
+#pragma unroll
+            for (int g = 0; g < CW; g++) c1[g] = c2[g] = make_float2(0.0f, 0.0f);
+        } else {
+            float2 ct = fp[r];
+            if (A.flag) {  // grid.cpp:267-269
+                const float arg = (float)j * offset;
+                const float phr = cosf(arg), phi = -sinf(arg);
+                ct = make_float2(ct.x * phr - ct.y * phi, ct.x * phi + ct.y * phr);
+            }
+            const float ci = -ct.y;
+#pragma unroll
+            for (int g = 0; g < CW; g++) {
+                const float2 P = lds[(long)g * A.pdim + gr_lds_idx(j)];
+                const float2 Q = lds[(long)g * A.pdim + gr_lds_idx(A.pdim - j)];
+                c1[g] = make_float2(ct.x * P.x - ct.y * P.y, ct.x * P.y + ct.y * P.x);
+                c2[g] = make_float2(ct.x * Q.x - ci * Q.y, ct.x * Q.y + ci * Q.x);
+            }
         }
         const long o = (long)j * kSpecGroup;
-        float2 ct = fp[r];
-        if (A.flag) {  // grid.cpp:267-269
-            const float arg = (float)j * offset;
-            const float phr = cosf(arg), phi = -sinf(arg);
-            ct = make_float2(ct.x * phr - ct.y * phi, ct.x * phi + ct.y * phr);
+        if (CW == 1) {
+            o1[o] = c1[0];
+            o2[o] = c2[0];
+        } else {
+#pragma unroll
+            for (int g = 0; g < CW; g += 2) {
+                reinterpret_cast<float4 *>(o1 + o + g)[0] = make_float4(c1[g].x, c1[g].y, c1[g + 1].x, c1[g + 1].y);
+                reinterpret_cast<float4 *>(o2 + o + g)[0] = make_float4(c2[g].x, c2[g].y, c2[g + 1].x, c2[g + 1].y);
+            }
         }
-        const float2 P = row[gr_lds_idx(j)], Q = row[gr_lds_idx(A.pdim - j)];
-        o1[o] = make_float2(ct.x * P.x - ct.y * P.y, ct.x * P.y + ct.y * P.x);
-        const float ci = -ct.y;
-        o2[o] = make_float2(ct.x * Q.x - ci * Q.y, ct.x * Q.y + ci * Q.x);
     }
 }
 
 }  // namespace
+
+// Pairs per workgroup: kRowPairs side by side when they fit a workgroup
+// (<= 1024 threads) and the CU's LDS (160 KiB) and the launch has that many
+// pairs, else fewer (a power of two that divides the 8 slots of a record).
+constexpr int kRowPairs = 2;
+
+static int rowfft_pairs_per_block(long pdim, int pairs) {
+    int cw = kRowPairs;
+    while (cw > 1 && (cw > pairs || (long)cw * gr_fft_threads(pdim) > 1024 || (long)cw * pdim * 8 > 163840)) cw /= 2;
+    return cw;
+}
 
 hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, const float *sino,
                             long slice_stride, long row_stride, const GrPre *pre, int n_slices, int pair0,
@@ -195,13 +235,20 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     A.spec2 = spec2;
     const int nthr = gr_fft_threads(g.pdim);
     const long per = (g.pdim / 4 + nthr - 1) / nthr;
-    const dim3 grid((unsigned)((n_ang + 7) / 8 * 8 * pairs)), block(nthr);
-    const size_t lds = sizeof(float2) * gr_lds_len(g.pdim);
-#define GR_LAUNCH_P(MB, PR)                                             \
-    do {                                                                \
-        hipError_t e_ = gr_allow_lds(k_rowfft<MB, PR>, lds);            \
-        if (e_ != hipSuccess) return e_;                                \
-        hipLaunchKernelGGL((k_rowfft<MB, PR>), grid, block, lds, s, A); \
+    const int cw = rowfft_pairs_per_block(g.pdim, pairs);
+    const dim3 grid((unsigned)((n_ang + 7) / 8 * 8 * ((pairs + cw - 1) / cw))), block(cw * nthr);
+    const size_t lds = sizeof(float2) * cw * gr_lds_len(g.pdim);
+#define GR_LAUNCH_C(MB, PR, CW)                                              \
+    do {                                                                     \
+        hipError_t e_ = gr_allow_lds(k_rowfft<MB, PR, CW>, lds);             \
+        if (e_ != hipSuccess) return e_;                                     \
+        hipLaunchKernelGGL((k_rowfft<MB, PR, CW>), grid, block, lds, s, A);  \
+    } while (0)
+#define GR_LAUNCH_P(MB, PR)                                                  \
+    do {                                                                     \
+        if (cw == 4) GR_LAUNCH_C(MB, PR, 4);                                 \
+        else if (cw == 2) GR_LAUNCH_C(MB, PR, 2);                            \
+        else GR_LAUNCH_C(MB, PR, 1);                                         \
     } while (0)
 #define GR_LAUNCH(MB)                                                   \
     do {                                                                \
@@ -219,6 +266,7 @@ hipError_t gr_launch_rowfft(const GrGeom &g, const GrDevTables &t, int n_ang, co
     case 16: GR_LAUNCH(16); break;
     default: return hipErrorInvalidValue;
     }
+#undef GR_LAUNCH_C
 #undef GR_LAUNCH
 #undef GR_LAUNCH_P
     return hipGetLastError();
