@@ -99,6 +99,32 @@ __host__ __device__ __forceinline__ bool gr_tile_active(int tx, int ty, float M2
     return sqrtf(cu * cu + cv * cv) <= rmax;
 }
 
+// The tiles of tile row ty (ntx tiles wide) that gr_tile_active admits:
+// [lo, hi], empty when lo > hi.  The predicate is monotone in |cu| (cu is
+// exact; the rounded square, sum and square root are monotone), so these
+// tiles form one interval around the column nearest the centre; a first
+// guess from the real-valued circle is moved to the interval's exact ends by
+// the predicate itself.  The row transform reads only these tiles.
+__host__ __device__ __forceinline__ void gr_active_tile_range(int ty, int ntx, float M2, float rmax, int &lo,
+                                                              int &hi) {
+    const float c = (M2 - 3.5f) / 8.0f;  // |cu| is smallest at tile column c
+    const int t0 = min(max((int)floorf(c), 0), ntx - 1), t1 = min(t0 + 1, ntx - 1);
+    const int tc = gr_tile_active(t0, ty, M2, rmax) ? t0 : (gr_tile_active(t1, ty, M2, rmax) ? t1 : -1);
+    if (tc < 0) {
+        lo = 1;
+        hi = 0;
+        return;
+    }
+    const float cv = (float)(ty * 8) + 3.5f - M2;
+    const float w = sqrtf(fmaxf(rmax * rmax - cv * cv, 0.0f)) / 8.0f;
+    lo = max(0, min(tc, (int)floorf(c - w)));
+    hi = min(ntx - 1, max(tc, (int)ceilf(c + w)));
+    while (lo < tc && !gr_tile_active(lo, ty, M2, rmax)) lo++;
+    while (lo > 0 && gr_tile_active(lo - 1, ty, M2, rmax)) lo--;
+    while (hi > tc && !gr_tile_active(hi, ty, M2, rmax)) hi--;
+    while (hi < ntx - 1 && gr_tile_active(hi + 1, ty, M2, rmax)) hi++;
+}
+
 // Angle window of an 8 x 8 tile with origin (tx*8, ty*8): the ranges
 // [r0, r1) and [r2, r3) of sorted angle keys (akey, ascending in [0, pi)) whose
 // sample line passes within dmax of the tile centre.  A line of direction

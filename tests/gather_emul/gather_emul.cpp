@@ -148,4 +148,28 @@ int gather_emul(int M, int half, int n_ang, float scale, float L2, float tblspcg
     return 0;
 }
 
+// For every tile row of an M x M grid: does gr_active_tile_range give exactly
+// the tiles gr_tile_active admits?  Returns the number of rows where it does
+// not (0 expected); *n_active receives the admitted tiles in all.
+int tile_range_check(int M, float rmax, long *n_active) {
+    const int ntx = M / 8;
+    const float M2 = (float)(M >> 1);
+    int bad = 0;
+    long count = 0;
+    for (int ty = 0; ty < ntx; ty++) {
+        int lo, hi;
+        gr_active_tile_range(ty, ntx, M2, rmax, lo, hi);
+        for (int tx = 0; tx < ntx; tx++) {
+            const bool act = gr_tile_active(tx, ty, M2, rmax);
+            count += act;
+            if (act != (tx >= lo && tx <= hi)) {
+                bad++;
+                break;
+            }
+        }
+    }
+    *n_active = count;
+    return bad;
+}
+
 }  // extern "C"

@@ -85,3 +85,24 @@ def test_gather_cfg2_geometry():
     Hg, Hs, tg, ts = run(g, seed=3)
     assert tg == ts
     assert np.abs(Hg - Hs).max() <= 1e-12 * np.abs(Hs).max()
+
+
+def test_active_tile_interval_is_exact():
+    """The row transform reads, per tile row, the interval of tiles that
+    gr_active_tile_range returns instead of testing every tile: for every row
+    of grids from 64 to 16384 cells and radii from 0 past the corners, the
+    interval holds exactly the tiles gr_tile_active admits (a tile missed
+    would drop gridded data, one too many would read stale memory)."""
+    lib = C.CDLL(LIB)
+    lib.tile_range_check.argtypes = [C.c_int, C.c_float, C.POINTER(C.c_long)]
+    rng = np.random.default_rng(4)
+    n = C.c_long()
+    for M in (64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384):
+        radii = [0.0, 0.5, 3.5, 4.0, 7.9, M / 2 - 0.5, M / 2 + 2.4, 0.75 * M, 0.8 * M]
+        radii += list(rng.uniform(0.0, 0.75 * M, 12 if M <= 4096 else 3))
+        counts = []
+        for r in radii:
+            bad = lib.tile_range_check(M, C.c_float(r), C.byref(n))
+            assert bad == 0, (M, r, bad)
+            counts.append(n.value)
+        assert counts[0] == 0 and counts[8] == (M // 8) ** 2, (M, counts[0], counts[8])  # r = 0, past the corners
