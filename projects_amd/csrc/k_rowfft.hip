@@ -1,4 +1,4 @@
-// k_rowfft.hip -- K1: one workgroup per (slice pair, angle).
+// k_rowfft.hip -- K1: one workgroup per (two slice pairs, angle).
 //
 // Replaces phase 1a/1b of do_recon (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:240-274):
 //   cproj[j] = (G1[n][j], G2[n][j]) for j < n_det, 0 up to pdim   (grid.cpp:240-252)
@@ -199,6 +199,8 @@ __global__ void __launch_bounds__(1024) k_rowfft(RowFftArgs A) {
 // Pairs per workgroup: kRowPairs side by side when they fit a workgroup
 // (<= 1024 threads) and the CU's LDS (160 KiB) and the launch has that many
 // pairs, else fewer (a power of two that divides the 8 slots of a record).
+// At pdim = 4096 on MI355X, per launch of 8 pairs: 0.33 ms with one pair per
+// workgroup, 0.20 with two, 0.27 with four.
 constexpr int kRowPairs = 2;
 
 static int rowfft_pairs_per_block(long pdim, int pairs) {
