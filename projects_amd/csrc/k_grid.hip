@@ -438,12 +438,12 @@ __device__ __forceinline__ void gop_consume(uint2 t, const float2 (&v)[P], float
 
 constexpr int kHeavyBatch = 16;  // rows loaded at once on the one-pair path
 
-// The one-pair-per-wave path for the few tiles near the centre of the grid
-// whose cells take thousands of taps: the P pairs of a group go to P
-// wavefronts, which stream the tile's rows kHeavyBatch at a time (all entry
-// loads, then all sample loads, then the adds), so each wave has many loads in
-// flight instead of a chain of dependent ones.  Per cell and pair the float
-// operations are those of the wide path.
+// The one-pair-per-wave path for the tiles near the centre of the grid whose
+// cells take hundreds to thousands of taps: the P pairs of a group go to P
+// wavefronts, which stream the tile's rows kHeavyBatch at a time (the batch's
+// sample loads, then the next batch's entry loads, then the batch's adds), so
+// each wave has many loads in flight instead of a chain of dependent ones.
+// Per cell and pair the float operations are those of the wide path.
 template <int ACCUM>
 __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &O, int ti, int pair, int lane) {
     const unsigned tt = A.tiles[ti];
