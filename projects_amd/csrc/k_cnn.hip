@@ -72,6 +72,37 @@ __device__ __forceinline__ bool positive_bf16(uint16_t h) { return !(h & 0x8000u
 
 __device__ __forceinline__ __bf16 bf16_of(float f) { return __builtin_bit_cast(__bf16, to_bf16(f)); }
 
+// Stage a tile's activations with their one-pixel halo (HS x WS pixels of 32
+// bf16 channels, four 16-byte pieces each; zeros outside the image) from HBM
+// into LDS at pixel stride PSTR, by a workgroup of 256 threads: a thread's
+// loads go out in batches of BATCH, each batch issued before its LDS stores,
+// so the tile costs ceil(6 / BATCH) memory latencies instead of one per loop
+// trip (BATCH bounds the registers the staging holds).
+template <int BATCH>
+__device__ __forceinline__ void stage_tile(const uint16_t *__restrict__ x, uint16_t *sx, long ibase, int H, int W,
+                                           int y0, int x0, int tid) {
+    constexpr int N = HS * WS * 4, PER = (N + 255) / 256;
+#pragma unroll
+    for (int k0 = 0; k0 < PER; k0 += BATCH) {
+        uint4 v[BATCH];
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = tid + (k0 + k) * 256;
+            const int pix = i >> 2, ch = i & 3;
+            const int r = pix / WS, c = pix - r * WS;
+            const int gy = y0 + r - 1, gx = x0 + c - 1;
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (k0 + k < PER && i < N && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v[k] = *reinterpret_cast<const uint4 *>(x + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+        }
+#pragma unroll
+        for (int k = 0; k < BATCH; k++) {
+            const int i = tid + (k0 + k) * 256;
+            if (k0 + k < PER && i < N) *reinterpret_cast<uint4 *>(sx + (i >> 2) * PSTR + (i & 3) * 8) = v[k];
+        }
+    }
+}
+
 // The 1 -> 32 and 32 -> 1 layers on MFMA too (v_mfma_f32_16x16x32_bf16, the
 // lane layout of the 32 -> 32 layers: A row = lane & 15, B column = lane & 15,
 // k = 8 * (lane >> 4) + e; D column = lane & 15, rows 4 * (lane >> 4) + e).
@@ -159,15 +190,7 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
     tm.of(tile, img, y0, x0);
     const long ibase = (long)img * H * W;
     __syncthreads();  // the previous tile's reads of sx are done
-    for (int i = tid; i < HS * WS * 4; i += 256) {
-        const int pix = i >> 2, ch = i & 3;
-        const int r = pix / WS, c = pix - r * WS;
-        const int gy = y0 + r - 1, gx = x0 + c - 1;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-            v = *reinterpret_cast<const uint4 *>(x + ((ibase + (long)gy * W + gx) * CH + ch * 8));
-        *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
-    }
+    stage_tile<3>(x, sx, ibase, H, W, y0, x0, tid);
     __syncthreads();
 
     const int j = lane & 15, g = lane >> 4;
@@ -328,9 +351,21 @@ __global__ void __launch_bounds__(256) k_conv_in(const float *__restrict__ x, fl
         tm.of(tile, img, y0, x0);
         const long ibase = (long)img * H * W;
         __syncthreads();  // the previous tile's reads of sxs are done
-        for (int i = tid; i < HS * WS; i += 256) {
-            const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
-            sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
+        {  // both loads of a thread in flight before the stores
+            constexpr int PER = (HS * WS + 255) / 256;
+            float v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int i = tid + k * 256;
+                const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
+                v[k] = i < HS * WS && gy >= 0 && gy < H && gx >= 0 && gx < W ? x[ibase + (long)gy * W + gx] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int i = tid + k * 256;
+                const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
+                if (i < HS * WS) sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(v[k] / scale) : 0.0f;
+            }
         }
         __syncthreads();
 #pragma unroll 1
@@ -416,15 +451,7 @@ __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a
     const long ibase = (long)img * H * W;
     for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W5 + i]);
     if (tid == 0) sw[9 * CH] = prm[OFF_B5];
-    for (int i = tid; i < HS * WS * 4; i += 256) {
-        const int pix = i >> 2, ch = i & 3;
-        const int r = pix / WS, c = pix - r * WS;
-        const int gy = y0 + r - 1, gx = x0 + c - 1;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-            v = *reinterpret_cast<const uint4 *>(a4 + ((ibase + (long)gy * W + gx) * CH + ch * 8));
-        *reinterpret_cast<uint4 *>(sx + pix * PSTR + ch * 8) = v;
-    }
+    stage_tile<6>(a4, sx, ibase, H, W, y0, x0, tid);
     for (int i = tid; i < TH * TW; i += 256) sred[0][i] = sred[1][i] = 0.0f;
     __syncthreads();
     bf16x8 A[9];
