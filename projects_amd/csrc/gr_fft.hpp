@@ -93,9 +93,21 @@ __device__ __forceinline__ void gr_dft16(float2 (&v)[16]) {
     for (int c = 0; c < 4; c++) gr_dft4<SIGN>(v[4 * c], v[4 * c + 1], v[4 * c + 2], v[4 * c + 3]);
 }
 
-template <int SIGN, int MAXB>
-__device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr,
-                                           int max_radix = 16) {
+// No output hook: the transform ends in buf.
+struct GrFftToLds {
+    static constexpr bool enabled = false;
+    __device__ void operator()(int, float2) const {}
+};
+
+// Returns true when an enabled output hook `out` received the result instead
+// of buf: when the last pass is a radix-16 one, out(i, X[i]) is called for
+// every output i straight from the registers of that pass (no LDS store, no
+// barrier); otherwise the result is in buf as usual and false is returned.
+// Every output i is handed over exactly once; for a fixed pass slot the
+// threads hand over consecutive i.
+template <int SIGN, int MAXB, class Out = GrFftToLds>
+__device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr,
+                                           int max_radix = 16, Out out = Out()) {
     int Ns = 1;
     const float2 *__restrict__ twp = tw + (max_radix >= 16 ? 0 : gr_twp_len(N, 16));
     constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
@@ -115,6 +127,19 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
                 }
                 gr_dft16<SIGN>(v[c]);
             }
+        }
+        if (Out::enabled && Ns * 16 == N) {  // the last pass: outputs straight from registers
+#pragma unroll
+            for (int c = 0; c < MAXB16; c++) {
+                const int j = tid + c * nthr;
+                if (j < nb16) {
+                    const int k = j & (Ns - 1);
+                    const int base = ((j - k) << 4) + k;
+#pragma unroll
+                    for (int q = 0; q < 16; q++) out(base + q * Ns, v[c][4 * (q & 3) + (q >> 2)]);
+                }
+            }
+            return true;
         }
         __syncthreads();
 #pragma unroll
@@ -191,6 +216,7 @@ __device__ __forceinline__ void gr_lds_fft(float2 *buf, int N, const float2 *__r
         }
         __syncthreads();
     }
+    return false;
 }
 
 // Threads per FFT block and the matching MAXB for a row of N points.

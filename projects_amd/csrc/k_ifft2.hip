@@ -32,6 +32,20 @@ struct RowsArgs {
     float rmax;  // tiles beyond it were not written by the gridding step: zeros
 };
 
+// Output hook of the row transform: FFT output i lands in window column
+// jj = j' - out_off, where corner_index(j') = i (outputs that map to no
+// window column are dropped).
+struct WindowRow {
+    float2 *dst;
+    int M, m02, off, out_n;
+    static constexpr bool enabled = true;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        const int jp = i >= M - m02 ? i - (M - m02) : i + m02;
+        const int jj = jp - off;
+        if (jj >= 0 && jj < out_n) dst[jj] = v;
+    }
+};
+
 // Loads and LDS stores in loops unrolled to the per-thread bound (M <= 4 MAXB
 // nthr), every load issued before the first store: one memory latency per
 // row instead of one per element.  CW rows per workgroup, transformed side by
@@ -60,9 +74,11 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
         if (i < M) row[gr_lds_idx(i)] = v[r];
     }
     __syncthreads();
-    gr_lds_fft<-1, MAXB>(row, M, A.tw, tid, nthr, A.radix);
     float2 *dst = A.z + ((long)p * M + b) * A.out_n;
     const int out_n = (int)A.out_n, off = (int)A.out_off, m02 = (int)A.M02;
+    // when the last pass is radix 16, the window's outputs go to HBM straight
+    // from its registers: output i is window column j' = corner_index^-1(i)
+    if (gr_lds_fft<-1, MAXB>(row, M, A.tw, tid, nthr, A.radix, WindowRow{dst, M, m02, off, out_n})) return;
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
         const int jj = tid + r * nthr;
@@ -83,6 +99,27 @@ struct ColsArgs {
     long out_slice_stride, out_row_stride;
     int M, cw, n_slices, pair0, radix;
     long M02, out_off, out_n;
+};
+
+// Output hook of the column transform: FFT output i is image column
+// kk = k' - out_off of this row, corner_index(k') = i, scaled by the
+// deapodization cu * winv[k'] (grid.cpp:401); rows past the window have
+// r1 = nullptr and write nothing.
+struct WindowCol {
+    float *r1, *r2;
+    const float *winv;
+    float cu;
+    int M, m02, off, out_n;
+    static constexpr bool enabled = true;
+    __device__ __forceinline__ void operator()(int i, float2 v) const {
+        const int kp = i >= M - m02 ? i - (M - m02) : i + m02;
+        const int kk = kp - off;
+        if (r1 && kk >= 0 && kk < out_n) {
+            const float corrn = cu * winv[kp];
+            r1[kk] = corrn * v.x;
+            if (r2) r2[kk] = corrn * v.y;
+        }
+    }
 };
 
 // Column group of a workgroup.  Workgroups are dealt to the 8 XCDs round
@@ -140,14 +177,22 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
         }
     }
     __syncthreads();
-    if (CW)
-        gr_lds_fft<-1, MAXB>(cols + grp * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
-    else
-        for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
-
     const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
     float *o1 = A.out + s1 * A.out_slice_stride;
     float *o2 = s2 < A.n_slices ? A.out + s2 * A.out_slice_stride : nullptr;
+    if (CW) {
+        // when the last pass is radix 16, each group writes its image row
+        // straight from that pass's registers (deapodization on the way)
+        const long jj = c0 + grp;
+        const bool live = jj < A.out_n;
+        WindowCol wc{live ? o1 + jj * A.out_row_stride : nullptr,
+                     live && o2 ? o2 + jj * A.out_row_stride : nullptr,
+                     A.winv, live ? A.winv[A.out_off + jj] : 0.0f, M, (int)A.M02, (int)A.out_off, out_n};
+        if (gr_lds_fft<-1, MAXB>(cols + grp * gr_lds_len(M), M, A.tw, tid, nthr, A.radix, wc)) return;
+    } else {
+        for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
+    }
+
     for (int c = CW ? grp : 0; c < (CW ? grp + 1 : cw); c++) {
         const long jj = c0 + c;
         if (jj >= A.out_n) break;
