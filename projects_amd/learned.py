@@ -238,9 +238,11 @@ def train(net: Corrector, inp, tgt, steps: int = 0, seconds: float = 0.0, batch:
     """Adam steps on random batches of (inp, tgt); stops after `steps` steps or
     `seconds` of wall time, whichever is set (both: the first reached).  The
     learning rate follows a cosine from lr down to lr * lr_final over the
-    budget (the larger of the step and time fractions used), so the weights
-    the run ends with are a settled iterate, not one of the noisy ones a
-    constant rate leaves (lr_final = 1: constant).
+    budget -- the step budget when `steps` is set (the time then only caps
+    the run, so the schedule, and with it the trained network, does not
+    depend on timing), else the time budget -- so the weights the run ends
+    with are a settled iterate, not one of the noisy ones a constant rate
+    leaves (lr_final = 1: constant).
     Returns the loss history [(step, loss)] sampled every log_every steps."""
     import math
     import time
@@ -256,7 +258,7 @@ def train(net: Corrector, inp, tgt, steps: int = 0, seconds: float = 0.0, batch:
     while True:
         idx = torch.randint(0, n, (batch,), device="cuda", generator=g)
         xb, tb = inp.index_select(0, idx).contiguous(), tgt.index_select(0, idx).contiguous()
-        f = max(k / steps if steps else 0.0, (time.perf_counter() - t0) / seconds if seconds else 0.0)
+        f = k / steps if steps else ((time.perf_counter() - t0) / seconds if seconds else 0.0)
         rate = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * min(f, 1.0))))
         loss = net.train_step(xb, tb, lr=rate)
         k += 1
