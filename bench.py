@@ -358,8 +358,9 @@ def bench_precision(a, gr, O, torch):
 def bench_learned(a, gr, torch):
     """Learned sparse-angle path.  Training: 512^2 seeded phantoms (20 ellipses
     + band-limited texture, noise-free), gridrec from 90 angles as input and
-    from 900 angles as target, Adam (lr 1e-3, batch 32) for a time budget on
-    the GPU.  Inference: 2048^2 slices from 150-angle sinograms, gridrec + the
+    from 900 angles as target, Adam (batch 32, learning rate on a cosine from
+    1e-3 to 5e-5) for a fixed number of steps on the GPU (a time cap bounds
+    the run).  Inference: 2048^2 slices from 150-angle sinograms, gridrec + the
     network (with the fallback), judged by PSNR / rel-L2 against 1500-angle
     plain gridrec of the same phantoms; both paths timed with HIP events."""
     from projects_amd import learned as Lrn
