@@ -6,7 +6,14 @@
 * FFT known answers and numpy's FFT;
 * properties the mathematics fixes: zero -> zero, linearity, the two-slice
   packing (grid.cpp:240-246, 272-274), centre-shift equivariance, and the
-  reconstruction of an analytically projected ellipse phantom.
+  reconstruction of an analytically projected ellipse phantom;
+* the exact-math FP64 variant: the same properties plus rotation equivariance,
+  which only exact angles satisfy;
+* the threaded batch driver: bit-equal to the pairwise restatement;
+* the mixed-precision oracle: each stage bit identified by a property no other
+  stage has (ACCUM order-dependent, TABLES linear, DEAPOD pointwise-bounded,
+  ROWFFT exact on impulse rows), and their rounding errors adding up in
+  variance to the all-FP32 error.
 """
 import numpy as np
 import pytest
@@ -204,3 +211,193 @@ def test_recovers_ellipse_phantom():
     slope, offset = np.linalg.lstsq(np.vstack([img[inside], np.ones(inside.sum())]).T,
                                     rec[inside], rcond=None)[0]
     assert abs(slope - 1) < 0.07 and abs(offset) < 0.1, (slope, offset)
+
+
+# ---------------------------------------------------------------- exact-math FP64 variant
+
+def test_exact64_zero_linearity_packing():
+    """or_recon_exact64: zero -> zero, linearity, and the two-slice packing."""
+    g = _g(center=31.6, filter="hann")
+    z = np.zeros((32, 64), np.float32)
+    s1, s2 = O.recon_f64(g, z, z, exact=True)
+    assert not s1.any() and not s2.any()
+    rng = np.random.default_rng(11)
+    a, b, c, d = rng.integers(-1000, 1000, (4, 32, 64)).astype(np.float32)
+    x1, x2 = O.recon_f64(g, a, b, exact=True)
+    y1, y2 = O.recon_f64(g, c, d, exact=True)
+    z1, z2 = O.recon_f64(g, (2 * a - 3 * c).astype(np.float32), (2 * b - 3 * d).astype(np.float32), exact=True)
+    assert O.rel_l2(z1, 2 * x1 - 3 * y1) < 1e-9 and O.rel_l2(z2, 2 * x2 - 3 * y2) < 1e-9
+    p1, _ = O.recon_f64(g, a, z, exact=True)
+    _, q2 = O.recon_f64(g, z, b, exact=True)
+    assert O.rel_l2(x1, p1) < 1e-9 and O.rel_l2(x2, q2) < 1e-9
+
+
+def test_exact64_rotation_equivariance():
+    """With exact angles (geom 2: theta_n = 2 pi n / n_ang) and n_ang a multiple
+    of 4, rolling the rows by n_ang/4 turns every sample by +90 degrees,
+    p'(t, theta) = p(t, theta - pi/2), so the object turns by +90 degrees:
+    f'(x, y) = f(y, -x), i.e. S'[j][k] = S[k][M0-1-j] = rot90(S) (j along x =
+    cos, k along y = sin).  The separable PSWF weight w(du) w(dv) and the box
+    clipping to [1, M-1] are symmetric under that turn, so the identity holds
+    to FP64 rounding for any rows.  The float angle recurrence of the reference
+    (trig_su, grid.cpp:544-560) breaks it, which the second check shows."""
+    n_ang, n_det = 32, 48
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=23.7, geom=2, filter="shlo")
+    rng = np.random.default_rng(3)
+    a, b = rng.standard_normal((2, n_ang, n_det)).astype(np.float32)
+    q = n_ang // 4
+    e1, e2 = O.recon_f64(g, a, b, exact=True)
+    r1, r2 = O.recon_f64(g, np.roll(a, q, axis=0), np.roll(b, q, axis=0), exact=True)
+    assert O.rel_l2(r1, np.rot90(e1)) < 1e-12 and O.rel_l2(r2, np.rot90(e2)) < 1e-12
+    assert O.rel_l2(r1, np.rot90(e1, 3)) > 0.5          # the turn has a direction
+    f1, _ = O.recon_f64(g, a, b)
+    h1, _ = O.recon_f64(g, np.roll(a, q, axis=0), np.roll(b, q, axis=0))
+    assert O.rel_l2(h1, np.rot90(f1)) > 1e-5            # float angles: not exact
+    # exact and float-angle variants differ only by the angle drift
+    assert 1e-6 < O.rel_l2(e1, f1) < 1e-2
+
+
+def test_exact64_center_shift_equivariance():
+    n_ang, n_det, d = 24, 96, 7
+    rng = np.random.default_rng(13)
+    core = rng.standard_normal((2, n_ang, 60)).astype(np.float32)
+    a = np.zeros((2, n_ang, n_det), np.float32)
+    b = np.zeros((2, n_ang, n_det), np.float32)
+    a[:, :, 10:70] = core
+    b[:, :, 10 + d:70 + d] = core
+    x1, x2 = O.recon_f64(O.Geometry(n_ang=n_ang, n_det=n_det, center=40.2, geom=2), a[0], a[1], exact=True)
+    y1, y2 = O.recon_f64(O.Geometry(n_ang=n_ang, n_det=n_det, center=40.2 + d, geom=2), b[0], b[1], exact=True)
+    assert O.rel_l2(y1, x1) < 1e-9 and O.rel_l2(y2, x2) < 1e-9
+
+
+def test_exact64_recovers_ellipse_phantom():
+    n, nang = 128, 400
+    sino = np.stack([phantom.sinogram(11, s, nang, n, n / 2, 8) for s in range(2)])
+    g = O.Geometry(n_ang=nang, n_det=n, center=n / 2).padded()
+    s1, _ = O.recon_f64(g, O.pad_half(sino[0]), O.pad_half(sino[1]), exact=True)
+    off = n // 2
+    rec = s1[off:off + n, off:off + n] * (n / 2)
+    img = phantom.phantom_image(11, 0, 8, n)
+    inside = np.hypot(*np.meshgrid(np.arange(n) - n / 2 + 0.5, np.arange(n) - n / 2 + 0.5)) < n / 2 * 0.9
+    slope, offset = np.linalg.lstsq(np.vstack([img[inside], np.ones(inside.sum())]).T, rec[inside], rcond=None)[0]
+    assert np.corrcoef(rec[inside], img[inside])[0, 1] > 0.97
+    assert abs(slope - 1) < 0.07 and abs(offset) < 0.1, (slope, offset)
+
+
+# ---------------------------------------------------------------- threaded batch driver
+
+@pytest.mark.parametrize("pad", [True, False])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_batch_f32_equals_pairwise(pad, threads):
+    """or_recon_batch_f32 (the CPU baseline's driver) is pad_half + recon_f32 +
+    crop per pair, bit for bit, whatever the thread count."""
+    n_ang, n_det, pairs = 20, 40, 3
+    sino = np.stack([phantom.sinogram(5, s, n_ang, n_det, 20.4, 6) for s in range(2 * pairs)])
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=20.4, filter="hamm")
+    gp = g.padded() if pad else g
+    M0 = O.sizes(gp)["M0"]
+    off, cn = (M0 - n_det) // 2, n_det
+    out = O.recon_batch_f32(g, sino, pad, off, cn, threads=threads)
+    for p in range(pairs):
+        r1, r2 = sino[2 * p], sino[2 * p + 1]
+        if pad:
+            r1, r2 = O.pad_half(r1), O.pad_half(r2)
+        s1, s2 = O.recon_f32(gp, r1, r2)
+        assert np.array_equal(out[2 * p], s1[off:off + cn, off:off + cn])
+        assert np.array_equal(out[2 * p + 1], s2[off:off + cn, off:off + cn])
+
+
+# ---------------------------------------------------------------- mixed-precision stages
+
+U32 = 2.0 ** -24
+STAGES = {"rowfft": O.MX_ROWFFT, "accum": O.MX_ACCUM, "fft2": O.MX_FFT2, "tables": O.MX_TABLES,
+          "deapod": O.MX_DEAPOD}
+
+
+@pytest.fixture(scope="module")
+def mixed_errors():
+    """Per stage s, the error of the pipeline with only s in FP32 (mask ALL & ~s)
+    against the FP64 oracle, on: integer rows (a, b), a second set (c, d), their
+    sum, and (a, b) with the angle list and rows permuted together."""
+    rng = np.random.default_rng(7)
+    n_ang, n_det = 24, 40
+    ang = np.sort(rng.uniform(0, 180, n_ang)).astype(np.float32)
+    perm = rng.permutation(n_ang)
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=20.3, geom=0, angles=ang, filter="hann")
+    gp = O.Geometry(n_ang=n_ang, n_det=n_det, center=20.3, geom=0, angles=ang[perm], filter="hann")
+    a, b, c, d = rng.integers(-500, 500, (4, n_ang, n_det)).astype(np.float32)
+    inputs = {"ab": (g, a, b), "cd": (g, c, d), "sum": (g, a + c, b + d), "perm": (gp, a[perm], b[perm])}
+    ref = {k: O.recon_f64(*v)[0] for k, v in inputs.items()}
+    err = {s: {k: O.recon_mixed(*v, O.MX_ALL & ~bit)[0] - ref[k] for k, v in inputs.items()}
+           for s, bit in STAGES.items()}
+    total = O.recon_mixed(g, a, b, 0)[0] - ref["ab"]
+    return err, ref, total
+
+
+def test_mixed_each_stage_contributes(mixed_errors):
+    err, ref, total = mixed_errors
+    for s, e in err.items():
+        r = np.linalg.norm(e["ab"]) / np.linalg.norm(ref["ab"])
+        assert 1e-9 < r < 1e-5, (s, r)
+    # independent rounding errors: their variances add up to the all-FP32 error's
+    ratio = np.linalg.norm(total) ** 2 / sum(np.linalg.norm(e["ab"]) ** 2 for e in err.values())
+    assert 0.5 < ratio < 2.0, ratio
+
+
+def test_mixed_accum_is_the_order_dependent_stage(mixed_errors):
+    """Permuting the angles (rows with them) reorders only the H accumulation
+    (grid.cpp:276-302): only the ACCUM stage's rounding error changes."""
+    err, _, _ = mixed_errors
+    for s, e in err.items():
+        change = np.linalg.norm(e["perm"] - e["ab"]) / np.linalg.norm(e["ab"])
+        assert (change > 0.3) if s == "accum" else (change < 1e-4), (s, change)
+
+
+def test_mixed_tables_is_the_linear_stage(mixed_errors):
+    """Float tables in an otherwise FP64 pipeline keep it linear, so their error
+    is linear in the rows; rounding errors of the other stages are not."""
+    err, _, _ = mixed_errors
+    for s, e in err.items():
+        resid = np.linalg.norm(e["sum"] - e["ab"] - e["cd"]) / np.linalg.norm(e["ab"])
+        assert (resid < 1e-6) if s == "tables" else (resid > 0.1), (s, resid)
+
+
+def test_mixed_deapod_is_the_pointwise_stage(mixed_errors):
+    """Phase 3 (grid.cpp:401-403) is two roundings per pixel: its error is at
+    most (2u + u^2)|S| pixel by pixel; every other stage spreads error."""
+    err, ref, _ = mixed_errors
+    for s, e in err.items():
+        worst = np.max(np.abs(e["ab"]) / (np.abs(ref["ab"]) * U32))
+        assert (worst <= 2.0 + 1e-6) if s == "deapod" else (worst > 10), (s, worst)
+
+
+def test_mixed_rowfft_exact_on_impulse_rows():
+    """Rows that are power-of-two impulses at detector 0 have the constant row
+    transform (exact in FP32) and, with float tables, filter products that are
+    exact: the ROWFFT bit then changes nothing, while on random rows it does."""
+    n_ang, n_det = 24, 40
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=19.6, filter="ramp")
+    imp = np.zeros((n_ang, n_det), np.float32)
+    imp[:, 0] = 2.0 ** (np.arange(n_ang) % 3)
+    z = np.zeros_like(imp)
+    for m in (0, O.MX_ACCUM, O.MX_FFT2, O.MX_DEAPOD, O.MX_ACCUM | O.MX_FFT2 | O.MX_DEAPOD):
+        x = O.recon_mixed(g, imp, z, m)
+        y = O.recon_mixed(g, imp, z, m | O.MX_ROWFFT)
+        assert np.array_equal(x[0], y[0]) and np.array_equal(x[1], y[1]), m
+        assert not np.array_equal(x[0], O.recon_mixed(g, imp, z, m ^ O.MX_ACCUM)[0]), m
+    rng = np.random.default_rng(9)
+    a = rng.standard_normal((n_ang, n_det)).astype(np.float32)
+    assert not np.array_equal(O.recon_mixed(g, a, z, 0)[0], O.recon_mixed(g, a, z, O.MX_ROWFFT)[0])
+
+
+def test_f64_tables_close_to_reference_tables():
+    """The FP64 tables (filphase, wtbl, winv in double) differ from the
+    reference's float tables, which are pinned bit-exact, by float rounding only:
+    the pipeline with float tables (mask ALL & ~TABLES) is within a few float
+    ulps of the FP64 oracle."""
+    g = O.Geometry(n_ang=30, n_det=64, center=33.1, filter="shlo", C=5.0)
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal((2, 30, 64)).astype(np.float32)
+    f64, _ = O.recon_f64(g, a, b)
+    ft, _ = O.recon_mixed(g, a, b, O.MX_ALL & ~O.MX_TABLES)
+    assert 1e-9 < O.rel_l2(ft, f64) < 1e-5
