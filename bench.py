@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--learned-steps", type=int, default=8000,
                     help="training steps (0 = time budget only); a fixed count makes the trained network reproducible")
     ap.add_argument("--learned-slices", type=int, default=256, help="inference slices (2048^2, 150 angles)")
-    ap.add_argument("--learned-truth-slices", type=int, default=4,
+    ap.add_argument("--learned-truth-slices", type=int, default=32,
                     help="inference slices also compared with the analytic phantom")
     ap.add_argument("--learned-threshold", type=float, default=0.5,
                     help="fallback when ||residual|| / ||image|| exceeds this")
@@ -90,6 +90,80 @@ def copy_calibration():
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_per_launch": 2 * src.numel() * 4, "launches": 4}), flush=True)
+
+
+def host_cpu():
+    """This host's CPU model (from /proc/cpuinfo), logical CPUs and those usable here."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "usable_cpus": len(os.sched_getaffinity(0))}
+
+
+def ref_worker(job):
+    """Child process: the reference's own do_recon (oracle/_ref, one global
+    context per process) on pairs of a saved batch; returns (slices, seconds)."""
+    path, pairs, n_ang, n_det, center = job
+    import oracle as O
+
+    data = np.load(path, mmap_mode="r")
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=center).padded()
+    t0 = time.perf_counter()
+    for p in pairs:
+        O.ref_recon(g, O.pad_half(data[2 * p]), O.pad_half(data[2 * p + 1]))
+    return 2 * len(pairs), time.perf_counter() - t0
+
+
+def cpu_baseline(a, O, sino, ns, nang, ndet, center, out_n):
+    """The reference's gridrec timed on this host: single core (one process,
+    2 pairs) and one process per usable core (up to 16, one pair each, run
+    together; the rate is the sum of the processes' own rates), with the
+    caller's half padding on the host as in tomo_recon.  Beside it the FP32
+    restatement (oracle/liboracle.so) threaded over all usable cores."""
+    import multiprocessing as mp
+    import tempfile
+
+    host = host_cpu()
+    procs = max(1, min(16, host["usable_cpus"]))
+    out = {"host": host}
+    if O.ref_available():
+        npairs = min(max(2, procs), ns // 2)
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "sino.npy")
+            np.save(path, sino[0:2 * npairs].cpu().numpy())
+            ctx = mp.get_context("spawn")
+            with ctx.Pool(1) as pool:
+                n1, t1 = pool.map(ref_worker, [(path, [0, 1], nang, ndet, center)])[0]
+            with ctx.Pool(procs) as pool:
+                res = pool.map(ref_worker, [(path, [w % npairs], nang, ndet, center) for w in range(procs)])
+        rate = sum(n / t for n, t in res)
+        out.update({"value": round(rate, 3), "unit": "slices/s", "cores": procs, "kind": "reference",
+                    "sample": f"{procs} processes x 1 pair ({nang} angles x {ndet} detectors, half-padded "
+                              f"to {2 * ndet} on the host) of GridrecCPP 5.2 do_recon compiled from its sources, "
+                              f"run together; {max(t for _, t in res):.1f} s",
+                    "single_core": {"value": round(n1 / t1, 4), "unit": "slices/s",
+                                    "sample": f"{n1} slices ({n1 // 2} pairs) in one process, {t1:.1f} s"}})
+    threads = procs
+    pairs = min(a.cpu_pairs or 4 * threads, ns // 2)   # ~5-10 s of CPU work at this size
+    s = sino[0:2 * pairs].cpu().numpy()
+    g = O.Geometry(n_ang=nang, n_det=ndet, center=center)
+    t0 = time.perf_counter()
+    O.recon_batch_f32(g, s, True, ndet // 2, out_n, threads=threads)
+    cdt = time.perf_counter() - t0
+    port = {"value": round(2 * pairs / cdt, 3), "unit": "slices/s", "cores": threads, "kind": "port",
+            "sample": f"{2 * pairs} slices ({pairs} pairs) of the same workload, FP32 restatement of "
+                      f"GridrecCPP, {threads} threads, {cdt:.1f} s"}
+    if "value" in out:
+        out["port_all_cores"] = port
+    else:
+        out.update(port)
+    return out
 
 
 def kernel_bytes(kind, n_ang, n_det, half, M, out_n, pairs):
@@ -432,13 +506,14 @@ def bench_learned(a, gr, torch):
     from projects_amd import phantom as Ph
     cc = (np.arange(n) - (n - 1) / 2.0) / (n / 2.0)
     disk = np.add.outer(cc * cc, cc * cc) < 0.95 ** 2
+    disk_t = torch.from_numpy(disk).cuda()
     truth_ps = {"plain1500": [], "plain150": [], "learned150": []}
-    for i in range(a.learned_truth_slices):
-        t = Ph.phantom_image(a.seed + 6, i, 20, n, n_tex=64, tex_amp=0.15, tex_fmax=24.0)[disk]
-        rngt = t.max() - t.min()
+    for i in range(min(a.learned_truth_slices, ns)):
+        t = Ph.phantom_image(a.seed + 6, i, 20, n, n_tex=64, tex_amp=0.15, tex_fmax=24.0, device="cuda")[disk_t]
+        rngt = float(t.max() - t.min())
         for key, img in (("plain1500", plain[i]), ("plain150", sparse[i]), ("learned150", corr[i])):
-            r = img.double().cpu().numpy()[disk] * (n / 2.0)
-            truth_ps[key].append(10.0 * np.log10(rngt * rngt / np.mean((r - t) ** 2)))
+            r = img.double()[disk_t] * (n / 2.0)
+            truth_ps[key].append(10.0 * math.log10(rngt * rngt / float(torch.mean((r - t) ** 2))))
     ps_sparse = [Lrn.psnr(sparse[i], plain[i]) for i in range(ns)]
     ps_learned = [Lrn.psnr(corr[i], plain[i]) for i in range(ns)]
     rl = lambda u, v: float(torch.linalg.norm((u - v).double()) / torch.linalg.norm(v.double()))
@@ -455,7 +530,12 @@ def bench_learned(a, gr, torch):
         "rel_l2_plain150_vs_plain1500": float(f"{np.mean([rl(sparse[i], plain[i]) for i in range(ns)]):.4e}"),
         "rel_l2_learned150_vs_plain1500": float(f"{np.mean([rl(corr[i], plain[i]) for i in range(ns)]):.4e}"),
         "psnr_vs_phantom": {k: round(float(np.mean(v)), 2) for k, v in truth_ps.items()},
-        "psnr_vs_phantom_slices": a.learned_truth_slices,
+        "psnr_vs_phantom_spread": {k: {"std": round(float(np.std(v)), 2), "min": round(float(np.min(v)), 2),
+                                       "max": round(float(np.max(v)), 2)} for k, v in truth_ps.items()},
+        "learned150_minus_plain1500_dB_vs_phantom_per_slice": {
+            "min": round(float(np.min(np.subtract(truth_ps["learned150"], truth_ps["plain1500"]))), 2),
+            "max": round(float(np.max(np.subtract(truth_ps["learned150"], truth_ps["plain1500"]))), 2)},
+        "psnr_vs_phantom_slices": len(truth_ps["plain1500"]),
         "learned150_minus_plain1500_dB_vs_phantom": round(float(np.mean(truth_ps["learned150"]) -
                                                             np.mean(truth_ps["plain1500"])), 2),
         "fallback_slices_of_first16": int((~used).sum()),
@@ -641,20 +721,13 @@ def main():
                                           "staging, row packing on 16 host threads (gr_exec_pairs)"}
         del h_sino, h_out
 
-    # CPU baseline: the oracle's FP32 restatement on this host's cores, bounded sample
+    # CPU baseline on this host, bounded samples of the same workload: the
+    # reference's own gridrec (oracle/_ref, GridrecCPP compiled from its
+    # sources) on one core and one process per core, and the FP32 restatement
+    # on all cores
     if not a.no_cpu:
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        pairs = a.cpu_pairs or 8 * threads          # ~10-20 s of CPU work at this size
-        pairs = min(pairs, ns // 2)
-        s = sino[0:2 * pairs].cpu().numpy()
-        g = O.Geometry(n_ang=nang, n_det=ndet, center=center)
-        t0 = time.perf_counter()
-        O.recon_batch_f32(g, s, True, ndet // 2, out_n, threads=threads)
-        cdt = time.perf_counter() - t0
-        result["cpu_baseline"] = {"value": round(2 * pairs / cdt, 3), "unit": "slices/s", "cores": threads,
-                                  "kind": "port",
-                                  "sample": f"{2 * pairs} slices ({pairs} pairs) of the same workload, FP32 "
-                                            f"restatement of GridrecCPP, {threads} threads, {cdt:.1f} s"}
+        progress("CPU baseline")
+        result["cpu_baseline"] = cpu_baseline(a, O, sino, ns, nang, ndet, center, out_n)
     plan.close()
     if a.workload == "all":
         del sino, out

@@ -18,8 +18,35 @@
  *   - recon_free() releases what recon_init set up (grid.cpp:418-434).
  * Where the reference exits (bad geometry, unknown C) these functions print
  * the reason to stderr and leave the outputs untouched; gridrec_last_error()
- * returns it.  All reconstruction runs on the GPU (HIP device 0, or the
+ * returns it (recon_init and do_recon clear it on entry, so after a call it
+ * is empty exactly when that call succeeded).  All reconstruction runs on the GPU (HIP device 0, or the
  * ordinal in the environment variable GRIDREC_DEVICE).
+ *
+ * Library: libgridrec_grid.so (link with -lgridrec_grid; it loads
+ * libgridrec_hip.so from its own directory).  The C-ABI library itself
+ * exports none of these C++ names or globals.
+ *
+ * Accounting of Gridrec5.2/grid.h line by line:
+ *   3-13   ANSI and system includes: kept (time.h and sys/stat.h are not
+ *          needed by any declaration here and are left to the caller)
+ *   16-22  macros max, min, free_matrix, abs, pi, TOLERANCE, LTBL_DEF: kept;
+ *          abs is the reference's float-safe macro (define
+ *          GRIDREC_NO_ABS_MACRO before including to keep std::abs usable)
+ *   24-58  complex, sg_struct, pswf_struct, grid_struct: same layout
+ *   61     verbose: defined by the library
+ *   68-73  get_parm, data_setup, get_sgram, put_image, usage: NOT provided --
+ *          the command-line driver's netCDF file I/O (grid_io.cpp), not part
+ *          of the reconstruction API; rel_sgram: provided
+ *   74-77  malloc_vector_f/_c, malloc_matrix_f/_c: provided
+ *   78-80  prn_gparm, prn_pswf, prn_sgparm: provided
+ *   83-91  get_pswf, get_filter, recon_init, do_recon (both forms),
+ *          recon_free: provided
+ *   94-95  four1, fourn: NOT provided -- the Numerical Recipes transforms
+ *          grid.cpp calls internally; every transform of do_recon runs in the
+ *          library's own GPU kernels
+ *   99-107 the 5.x globals: defined by the library
+ *   108-109 fshift, global_index: declared as in the reference, which also
+ *          only declares them (the application defines them)
  */
 #ifndef GRIDREC_DROPIN_GRID_H
 #define GRIDREC_DROPIN_GRID_H
@@ -38,6 +65,9 @@
 #define min(A, B) ((A) < (B) ? (A) : (B))
 #endif
 #define free_matrix(A) (free(*(A)), free(A))
+#if !defined(abs) && !defined(GRIDREC_NO_ABS_MACRO)
+#define abs(A) ((A) > 0 ? (A) : -(A))
+#endif
 #ifndef pi
 #define pi 3.14159265359
 #endif
@@ -85,6 +115,9 @@ float *malloc_vector_f(long n);
 complex *malloc_vector_c(long n);
 float **malloc_matrix_f(long nr, long nc);
 complex **malloc_matrix_c(long nr, long nc);
+void prn_gparm(grid_struct *G);
+void prn_pswf(pswf_struct *P);
+void prn_sgparm(sg_struct *S);
 
 /* pswf.cpp */
 void get_pswf(float C, pswf_struct **P);
@@ -118,6 +151,8 @@ extern complex *cproj, *filphase, **H;
 extern float **G1, **G2;
 extern float **S1, **S2;
 extern long imgsiz;
+extern FILE *fshift;     /* declared only, as in the reference */
+extern int global_index; /* declared only, as in the reference */
 
 #if defined(__GNUC__)
 #pragma GCC visibility pop

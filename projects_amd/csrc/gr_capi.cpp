@@ -232,11 +232,30 @@ std::vector<unsigned> sector_tiles(int nt, int ts, long M) {
     return out;
 }
 
+// The host-row entry points' staging slots, streams and events; every handle
+// is reset, so a partly built set can be freed and built again.
+void free_staging(gr_plan *pl) {
+    for (int b = 0; b < 2; b++) {
+        if (pl->h_in[b]) (void)hipHostFree(pl->h_in[b]);
+        if (pl->h_out[b]) (void)hipHostFree(pl->h_out[b]);
+        if (pl->d_in[b]) (void)hipFree(pl->d_in[b]);
+        if (pl->d_out[b]) (void)hipFree(pl->d_out[b]);
+        for (hipEvent_t e : {pl->ev_up[b], pl->ev_done[b], pl->ev_down[b]})
+            if (e) (void)hipEventDestroy(e);
+        pl->h_in[b] = pl->h_out[b] = pl->d_in[b] = pl->d_out[b] = nullptr;
+        pl->ev_up[b] = pl->ev_done[b] = pl->ev_down[b] = nullptr;
+    }
+    if (pl->up_stream) (void)hipStreamDestroy(pl->up_stream);
+    if (pl->down_stream) (void)hipStreamDestroy(pl->down_stream);
+    pl->up_stream = pl->down_stream = nullptr;
+    pl->stage_pairs = 0;
+}
+
 void free_plan(gr_plan *pl) {
     if (!pl) return;
+    free_staging(pl);
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
-                    pl->d_aord, pl->d_ang, pl->d_in[0], pl->d_in[1], pl->d_out[0], pl->d_out[1], pl->d_gop,
-                    pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl};
+                    pl->d_aord, pl->d_ang, pl->d_gop, pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -247,14 +266,6 @@ void free_plan(gr_plan *pl) {
         for (void *p : ws)
             if (p) (void)hipFree(p);
     }
-    for (int b = 0; b < 2; b++) {
-        if (pl->h_in[b]) (void)hipHostFree(pl->h_in[b]);
-        if (pl->h_out[b]) (void)hipHostFree(pl->h_out[b]);
-        for (hipEvent_t e : {pl->ev_up[b], pl->ev_done[b], pl->ev_down[b]})
-            if (e) (void)hipEventDestroy(e);
-    }
-    if (pl->up_stream) (void)hipStreamDestroy(pl->up_stream);
-    if (pl->down_stream) (void)hipStreamDestroy(pl->down_stream);
     for (auto &q : pl->pending) {
         (void)hipEventDestroy(q.e0);
         (void)hipEventDestroy(q.e1);
@@ -386,7 +397,9 @@ size_t gr_plan_workspace_bytes(const gr_plan *pl, int max_pairs) {
            up(sizeof(float2) * z_pp * max_pairs);
 }
 
-void gr_plan_bind_workspace(gr_plan *pl, void *base, int max_pairs) {
+int gr_plan_bind_workspace(gr_plan *pl, void *base, size_t bytes, int max_pairs) {
+    if (!base || max_pairs < 1 || bytes < gr_plan_workspace_bytes(pl, max_pairs))
+        return fail(GR_EINVAL, "workspace smaller than the plan needs for its pairs per launch");
     const GrGeom &g = pl->g;
     const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -401,6 +414,7 @@ void gr_plan_bind_workspace(gr_plan *pl, void *base, int max_pairs) {
     pl->d_z = (float2 *)p;
     pl->max_pairs = max_pairs;
     pl->owns_workspace = false;
+    return GR_OK;
 }
 
 int gr_plan_build_operator(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
@@ -676,19 +690,27 @@ int exec_pairs(gr_plan *pl, const float *const *G1, const float *const *G2, floa
     const size_t in_pp = (size_t)2 * n_ang * nd, out_pp = (size_t)2 * on * on;  // floats per pair
     if (!pl->stage_pairs) {
         const int sp = pl->max_pairs;
-        for (int b = 0; b < 2; b++) {
-            GR_HIP(hipHostMalloc((void **)&pl->h_in[b], sizeof(float) * in_pp * sp));
-            GR_HIP(hipHostMalloc((void **)&pl->h_out[b], sizeof(float) * out_pp * sp));
-            int rc = dev_alloc(pl, &pl->d_in[b], in_pp * sp);
-            if (rc) return rc;
-            rc = dev_alloc(pl, &pl->d_out[b], out_pp * sp);
-            if (rc) return rc;
-            GR_HIP(hipEventCreateWithFlags(&pl->ev_up[b], hipEventDisableTiming));
-            GR_HIP(hipEventCreateWithFlags(&pl->ev_done[b], hipEventDisableTiming));
-            GR_HIP(hipEventCreateWithFlags(&pl->ev_down[b], hipEventDisableTiming));
+        auto setup = [&]() -> int {
+            for (int b = 0; b < 2; b++) {
+                GR_HIP(hipHostMalloc((void **)&pl->h_in[b], sizeof(float) * in_pp * sp));
+                GR_HIP(hipHostMalloc((void **)&pl->h_out[b], sizeof(float) * out_pp * sp));
+                int rc = dev_alloc(pl, &pl->d_in[b], in_pp * sp);
+                if (rc) return rc;
+                rc = dev_alloc(pl, &pl->d_out[b], out_pp * sp);
+                if (rc) return rc;
+                GR_HIP(hipEventCreateWithFlags(&pl->ev_up[b], hipEventDisableTiming));
+                GR_HIP(hipEventCreateWithFlags(&pl->ev_done[b], hipEventDisableTiming));
+                GR_HIP(hipEventCreateWithFlags(&pl->ev_down[b], hipEventDisableTiming));
+            }
+            GR_HIP(hipStreamCreateWithFlags(&pl->up_stream, hipStreamNonBlocking));
+            GR_HIP(hipStreamCreateWithFlags(&pl->down_stream, hipStreamNonBlocking));
+            return GR_OK;
+        };
+        const int rc = setup();
+        if (rc) {  // a failure part-way frees what was built; the next call starts afresh
+            free_staging(pl);
+            return rc;
         }
-        GR_HIP(hipStreamCreateWithFlags(&pl->up_stream, hipStreamNonBlocking));
-        GR_HIP(hipStreamCreateWithFlags(&pl->down_stream, hipStreamNonBlocking));
         pl->stage_pairs = sp;
     }
     const int sp = pl->stage_pairs, nch = (npairs + sp - 1) / sp;

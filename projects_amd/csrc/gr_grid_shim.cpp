@@ -1,4 +1,6 @@
-// gr_grid_shim.cpp -- the reference's C++ API (include/grid.h) over the C-ABI.
+// gr_grid_shim.cpp -- the reference's C++ API (include/grid.h) over the C-ABI,
+// built into its own library (libgridrec_grid.so) so that the C-ABI library
+// exports none of the reference's generic global names (M, L, H, flag, ...).
 //
 // One process-wide plan, as the reference keeps one set of globals
 // (gridrec/GridrecCPP/Gridrec5.2/grid.cpp:28-37): recon_init creates it,
@@ -16,6 +18,7 @@
 #include "../../include/grid.h"
 #undef max
 #undef min
+#undef abs
 
 // ---- globals of the 5.x API ------------------------------------------------
 int verbose = 0;
@@ -110,6 +113,24 @@ complex **malloc_matrix_c(long nr, long nc) {
     return m;
 }
 
+// ---- grid_io.cpp:405-430: parameter listings --------------------------------
+void prn_gparm(grid_struct *G) {
+    printf("Contents of grid_struct:\n\t");
+    prn_pswf(G->pswf);
+    printf("\n  sampl=%f MaxPixSiz=%f R=%f X0=%f Y0=%f\n  filter=%s ltbl=%ld\n", G->sampl, G->MaxPixSiz, G->R,
+           G->X0, G->Y0, G->fname, G->ltbl);
+}
+
+void prn_pswf(pswf_struct *P) {
+    printf("Contents of pswf: C=%f nt=%d lmbda=%f\n\tCoefficients:\n", P->C, P->nt, P->lmbda);
+    for (int i = 0; i < P->nt / 2 + 1 && i < 15; i++) printf("\t\t%f\n", P->coefs[i]);
+}
+
+void prn_sgparm(sg_struct *S) {
+    printf("Contents of sg_struct:\n\tn_ang=%d n_det=%d geom=%d center=%f\n", S->n_ang, S->n_det, S->geom,
+           S->center);
+}
+
 // ---- pswf.cpp / filters.cpp ---------------------------------------------------
 void get_pswf(float C, pswf_struct **P) {
     load_db();
@@ -165,6 +186,14 @@ void recon_free(void) {
 
 void recon_init(grid_struct *GP, sg_struct *SGP, long *im_size) {
     recon_free();
+    g_last.clear();
+    if (!GP || !SGP || !GP->pswf) {
+        // grid.cpp:80 reads GP->pswf without a check; get_pswf exits on an
+        // unknown C (pswf.cpp:39-44), so the reference never gets here
+        g_last = "recon_init: no grid_struct, sg_struct or PSWF record (get_pswf failed?)";
+        fprintf(stderr, "gridrec: %s\n", g_last.c_str());
+        return;
+    }
     gr_params p;
     memset(&p, 0, sizeof p);
     p.n_ang = SGP->n_ang;
@@ -172,13 +201,13 @@ void recon_init(grid_struct *GP, sg_struct *SGP, long *im_size) {
     p.geom = SGP->geom;
     p.angles = SGP->angles;
     p.center = SGP->center;
-    p.C = GP->pswf ? GP->pswf->C : 6.0f;
+    p.C = GP->pswf->C;
     p.sampl = GP->sampl;
     p.MaxPixSiz = GP->MaxPixSiz;
     p.R = GP->R;
     p.X0 = GP->X0;
     p.Y0 = GP->Y0;
-    strncpy(p.filter, GP->fname, 15);
+    snprintf(p.filter, sizeof p.filter, "%.15s", GP->fname);
     p.ltbl = GP->ltbl;
     p.pad = GR_PAD_NONE;
     p.accum = GR_ACCUM_COMPENSATED;
@@ -187,13 +216,11 @@ void recon_init(grid_struct *GP, sg_struct *SGP, long *im_size) {
     p.device = dev ? atoi(dev) : 0;
     p.max_pairs = 1;
     p.filter_fn = GP->filter;
-    if (GP->pswf) {
-        g_pswf_copy.C = GP->pswf->C;
-        g_pswf_copy.nt = GP->pswf->nt;
-        g_pswf_copy.lmbda = GP->pswf->lmbda;
-        memcpy(g_pswf_copy.coefs, GP->pswf->coefs, sizeof g_pswf_copy.coefs);
-        p.pswf = &g_pswf_copy;
-    }
+    g_pswf_copy.C = GP->pswf->C;
+    g_pswf_copy.nt = GP->pswf->nt;
+    g_pswf_copy.lmbda = GP->pswf->lmbda;
+    memcpy(g_pswf_copy.coefs, GP->pswf->coefs, sizeof g_pswf_copy.coefs);
+    p.pswf = &g_pswf_copy;
     if (gr_plan_create(&p, &g_plan) != GR_OK) {
         g_plan = nullptr;
         report("recon_init");
@@ -228,6 +255,7 @@ void recon_init(grid_struct *GP, sg_struct *SGP, long *im_size) {
 }
 
 void do_recon() {
+    g_last.clear();
     if (!g_plan) {
         g_last = "do_recon: recon_init did not succeed";
         fprintf(stderr, "gridrec: %s\n", g_last.c_str());
@@ -238,6 +266,7 @@ void do_recon() {
 }
 
 void do_recon(float **sgram1, float **sgram2, float ***image1, float ***image2) {
+    g_last.clear();
     if (!g_plan) {
         g_last = "do_recon: recon_init did not succeed";
         fprintf(stderr, "gridrec: %s\n", g_last.c_str());

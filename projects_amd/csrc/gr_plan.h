@@ -91,9 +91,9 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan, bool alloc_work
 // Workspace bytes for max_pairs slice pairs per launch (the spectra are held
 // in whole groups of kSpecGroup pairs).
 size_t gr_plan_workspace_bytes(const gr_plan *plan, int max_pairs);
-// Point the plan's workspace at base (gr_plan_workspace_bytes(plan, max_pairs)
-// bytes, 256-byte aligned), owned by the caller.
-void gr_plan_bind_workspace(gr_plan *plan, void *base, int max_pairs);
+// Point the plan's workspace at base (`bytes` long, 256-byte aligned), owned by
+// the caller; GR_EINVAL when bytes < gr_plan_workspace_bytes(plan, max_pairs).
+int gr_plan_bind_workspace(gr_plan *plan, void *base, size_t bytes, int max_pairs);
 int gr_set_error(int code, const char *msg);
 // Build the plan's gather operator now, on stream s (once: a plan that has
 // one, or declined one, keeps that state).  With required false it declines

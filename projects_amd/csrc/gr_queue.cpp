@@ -17,8 +17,11 @@
 //    branches fork from and join back into the caller's stream, so small
 //    datasets run side by side and the host issues one launch per batch.
 //
-// A queue's results are bitwise those of per-dataset gr_recon_device calls:
-// the same kernels, and no tuning knob changes any sum order.
+// A queue's results are bitwise those of per-dataset gr_recon_device calls
+// where the tuner keeps the default FFT radix (16): the same kernels, and the
+// pairs per launch, column width and gridding workgroup size change no sum
+// order.  A class tuned to radix 4 computes its FFTs in another order and
+// agrees with the per-dataset images to FP32 rounding.
 #include <algorithm>
 #include <map>
 #include <numeric>
@@ -215,7 +218,9 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         float best_ms = 0.0f;
         // one timing of the class's dataset with a candidate setting
         auto trial = [&](int mp, int cw, int radix, int wpb, float *ms) {
-            gr_plan_bind_workspace(p, tune_ws, mp);
+            // mp <= min(pairs, 8): tune_bytes covers it
+            int brc = gr_plan_bind_workspace(p, tune_ws, tune_bytes, mp);
+            if (brc) return brc;
             p->cols_per_block = cw;
             p->g.fft_radix = radix;
             p->g.grid_wpb = wpb;
@@ -264,7 +269,9 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
             return rc;
         }
         for (int d : kv.second) {
-            q->plans[d]->max_pairs = best_mp;
+            // a dataset with fewer pairs than the tuned count launches them all
+            // at once either way; its workspace holds only its own pairs
+            q->plans[d]->max_pairs = std::min(best_mp, std::max(1, (q->ds[d].n_slices + 1) / 2));
             q->plans[d]->cols_per_block = best_cw;
             q->plans[d]->g.fft_radix = best_radix;
             q->plans[d]->g.grid_wpb = best_wpb;
@@ -272,9 +279,31 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     }
     // one pass over every dataset with its tuned settings before capture: the
     // per-dataset costs for the stream balance, and every kernel instance the
-    // graph will hold has been launched (and its LDS limit raised) once
+    // graph will hold has been launched (and its LDS limit raised) once.  The
+    // tuned pairs per launch of a class can exceed what the tuning buffer was
+    // sized for on another member (more angles, fewer slices than the class's
+    // tuned dataset), so the buffer is grown to the largest tuned workspace.
+    {
+        size_t pass_bytes = 0;
+        for (int d = 0; d < n_datasets; d++)
+            pass_bytes = std::max(pass_bytes, gr_plan_workspace_bytes(q->plans[d], q->plans[d]->max_pairs));
+        if (pass_bytes > tune_bytes) {
+            (void)hipFree(tune_ws);
+            tune_ws = nullptr;
+            if (hipMalloc(&tune_ws, pass_bytes) != hipSuccess) {
+                free_queue(q);
+                return gr_set_error(GR_ENOMEM, "queue workspace allocation failed");
+            }
+            tune_bytes = pass_bytes;
+        }
+    }
     for (int d = 0; d < n_datasets; d++) {
-        gr_plan_bind_workspace(q->plans[d], tune_ws, q->plans[d]->max_pairs);
+        rc = gr_plan_bind_workspace(q->plans[d], tune_ws, tune_bytes, q->plans[d]->max_pairs);
+        if (rc) {
+            (void)hipFree(tune_ws);
+            free_queue(q);
+            return rc;
+        }
         float ms = 0.0f;
         QHIP_Q(hipEventRecord(q->events[0], s0));
         rc = run_dataset(q, d, s0);
@@ -312,7 +341,8 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         }
     }
     for (int d = 0; d < n_datasets; d++)
-        gr_plan_bind_workspace(q->plans[d], q->workspaces[q->stream_of[d]], q->plans[d]->max_pairs);
+        QCK(gr_plan_bind_workspace(q->plans[d], q->workspaces[q->stream_of[d]], ws_bytes[q->stream_of[d]],
+                                   q->plans[d]->max_pairs));
 
     // one graph: fork from stream 0, every dataset on its stream, join
     if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) != hipSuccess) {

@@ -293,23 +293,36 @@ def sinogram(seed: int, slice_idx: int, n_ang: int, n_det: int, center: float, n
 
 
 def phantom_image(seed: int, slice_idx: int, n_ell: int, n: int, density: int = DENSITY_UNIFORM, n_tex: int = 0,
-                  tex_amp: float = 0.0, tex_fmax: float = 0.0) -> np.ndarray:
+                  tex_amp: float = 0.0, tex_fmax: float = 0.0, device=None):
     """The phantom itself on an n x n grid covering [-1, 1]^2 (pixel centres),
     row index = y... returned as img[x_index][y_index] to match gridrec's
-    S[j][k] with j along cos (x) and k along sin (y)."""
+    S[j][k] with j along cos (x) and k along sin (y).  With a torch device the
+    same float64 expressions are evaluated there and a float64 tensor is
+    returned (large images for the benchmark's comparisons)."""
     E = ellipses(seed, slice_idx, n_ell, density)
-    c = (np.arange(n) - (n - 1) / 2.0) / (n / 2.0)
-    X, Y = np.meshgrid(c, c, indexing="ij")
-    img = np.zeros((n, n))
+    if device is not None:
+        import torch
+
+        xp, kw = torch, dict(dtype=torch.float64, device=device)
+        c = (torch.arange(n, **kw) - (n - 1) / 2.0) / (n / 2.0)
+        X, Y = torch.meshgrid(c, c, indexing="ij")
+        img = torch.zeros((n, n), **kw)
+        where = lambda m, v: torch.where(m, torch.tensor(v, **kw), torch.tensor(0.0, **kw))
+    else:
+        xp = np
+        c = (np.arange(n) - (n - 1) / 2.0) / (n / 2.0)
+        X, Y = np.meshgrid(c, c, indexing="ij")
+        img = np.zeros((n, n))
+        where = lambda m, v: np.where(m, v, 0.0)
     for i in range(len(E.a)):
         ca, sa = math.cos(float(E.phi[i])), math.sin(float(E.phi[i]))
         dx, dy = X - float(E.x0[i]), Y - float(E.y0[i])
         xr = dx * ca + dy * sa
         yr = -dx * sa + dy * ca
-        img += np.where((xr / float(E.a[i])) ** 2 + (yr / float(E.b[i])) ** 2 <= 1.0, float(E.rho[i]), 0.0)
+        img += where((xr / float(E.a[i])) ** 2 + (yr / float(E.b[i])) ** 2 <= 1.0, float(E.rho[i]))
     if n_tex > 0:
         T = waves(seed, slice_idx, n_tex, tex_amp, tex_fmax)
-        tex = sum(float(T.amp[i]) * np.cos(2.0 * math.pi * (float(T.fx[i]) * X + float(T.fy[i]) * Y) + float(T.phase[i]))
-                  for i in range(n_tex))
-        img += np.where(X * X + Y * Y <= 1.0, tex, 0.0)
+        tex = sum(float(T.amp[i]) * xp.cos(2.0 * math.pi * (float(T.fx[i]) * X + float(T.fy[i]) * Y) +
+                                           float(T.phase[i])) for i in range(n_tex))
+        img += xp.where(X * X + Y * Y <= 1.0, tex, xp.zeros_like(tex))
     return img

@@ -183,3 +183,68 @@ def test_trained_network_beats_sparse_gridrec():
           flush=True)
     assert bool(used.all())
     assert learned > plain + 1.0
+
+
+def _window(a, y0, x0, w, halo):
+    """a[..., y0-halo : y0+w+halo, x0-halo : x0+w+halo] clipped to the image,
+    and the offsets of the w x w window inside it."""
+    H, W = a.shape[-2:]
+    ya, xa = max(0, y0 - halo), max(0, x0 - halo)
+    yb, xb = min(H, y0 + w + halo), min(W, x0 + w + halo)
+    return a[..., ya:yb, xa:xb], y0 - ya, x0 - xa
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_inference_2048_slice_matches_oracle(fused):
+    """The sparse-angle inference shape: one 2048 x 2048 gridrec image of a
+    150-angle, 2048-detector sinogram through the network on the GPU, checked
+    against the oracle on windows.  Five 3x3 layers see 5 pixels around each
+    output, so the oracle on a window widened by that halo (clipped at the
+    image's edges, where both sides pad with zeros) gives the window's exact
+    outputs; the windows include two image corners."""
+    from projects_amd import gridrec as gr
+
+    n, nang = 2048, 150
+    sino = torch.empty((1, nang, n), dtype=torch.float32, device="cuda")
+    gr.phantom_sinograms(sino, 77, nang, n, [n / 2], 20, n_tex=64, tex_amp=0.15, tex_fmax=24.0)
+    x = Lrn.gridrec_images(sino, n, n / 2)
+    assert x.shape == (1, n, n)
+    net, pd, _, _ = make(8, scale=float(x.std()))
+    y, _ = net.infer(x) if fused else net.forward(x)
+    torch.cuda.synchronize()
+    xs, ys = x.cpu().numpy(), y.cpu().numpy()
+    w, halo = 160, 5
+    for y0, x0 in ((0, 0), (n - w, n - w), (n // 2 - w // 2, n // 2 - w // 2), (311, 1500)):
+        xw, oy, ox = _window(xs, y0, x0, w, halo)
+        c = K.forward(pd, xw, net.scale)
+        ref = c["y"][:, oy:oy + w, ox:ox + w] * net.scale
+        e = rel(ys[:, y0:y0 + w, x0:x0 + w], ref)
+        print(f"window ({y0}, {x0}): rel-L2 vs oracle {e:.2e}", flush=True)
+        assert e < 1e-2, (y0, x0, e)
+
+
+def test_backward_on_512_gridrec_pairs():
+    """The training shape: two 512 x 512 gridrec pairs (90-angle input,
+    900-angle target of the same phantoms), forward and backward on the GPU,
+    against the oracle's backward on the GPU's activations."""
+    inp, tgt = Lrn.make_pairs(2, 512, 90, 900, seed=21)
+    net, pd, _, _ = make(9, scale=float(tgt.std()))
+    net.forward(inp, train=True)
+    net.backward(inp, tgt)
+    torch.cuda.synchronize()
+    x, t = inp.cpu().numpy(), tgt.cpu().numpy()
+    acts = [bf16_bits_to_f32(a) for a in net._buffers(*x.shape, True)[0]]
+    c = K.forward(pd, x, net.scale)
+    yo = c["y"]
+    for l in range(4):
+        c[f"a{l + 1}"] = acts[l]
+    c["y"] = net._buffers(*x.shape, True)[2].cpu().numpy()
+    assert rel(c["y"], yo) < 1e-2  # the forward, end to end against the oracle's BF16 chain
+    loss, g = K.loss_and_backward(pd, c, t, net.scale)
+    print(f"loss GPU {float(net.loss.cpu()):.6e} oracle {loss:.6e}", flush=True)
+    assert abs(float(net.loss.cpu()) - loss) <= 1e-5 * loss
+    gg = Lrn.unflatten(net.grads.cpu().numpy())
+    for k in g:
+        e = rel(gg[k], g[k])
+        print(f"grad {k}: rel-L2 {e:.2e}", flush=True)
+        assert e < 1e-2, k

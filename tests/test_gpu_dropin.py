@@ -1,0 +1,96 @@
+"""The drop-in C++ API (include/grid.h, libgridrec_grid.so) run in-process the
+way the reference's caller runs it (tests/dropin/grid_caller.cpp mirrors
+Multitomo_unit.txt:786-856): get_pswf -> get_filter -> recon_init -> do_recon
+per slice pair -> recon_free, in the 4.9 form (do_recon allocates the images)
+and the 5.x form (globals G1, G2, S1, S2).
+
+Bars: the images are bitwise those of the C-ABI plan with the same parameters
+(the same kernels), and against the reference's own gridrec -- its golden
+vectors and the cfg1 fixture -- they meet the bars of the other parity tests
+(1e-5 on the disk / crop, 1e-4 on whole M0 x M0 images).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import dropin_caller as D
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+if torch.cuda.is_available():
+    from projects_amd import gridrec as gr
+
+
+def geometry_of(case, arr, i):
+    c = {k: v for k, v in case.items() if k not in ("M0", "pdim", "M", "sampl_out", "L", "scale")}
+    return O.Geometry(angles=arr.get(f"case{i:02d}_angles"), **c)
+
+
+def disk(M0, frac=0.9):
+    yy, xx = np.mgrid[:M0, :M0]
+    return np.hypot(yy - M0 // 2, xx - M0 // 2) < frac * M0 / 2
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_dropin_golden_cases(golden, form):
+    meta, arr = golden
+    for i, case in enumerate(meta["cases"]):
+        g = geometry_of(case, arr, i)
+        a, b = arr[f"case{i:02d}_in"]
+        out, imgsiz, smp, rc, err = D.recon(form, np.stack([a, b]), g.n_det, g.center, geom=g.geom, angles=g.angles,
+                                            filter=g.filter, C_=g.C, sampl=g.sampl, R=g.R, MaxPixSiz=g.MaxPixSiz,
+                                            X0=g.X0, Y0=g.Y0, ltbl=g.ltbl, M0=case["M0"])
+        assert rc == 0 and err == "", (i, err)
+        assert imgsiz == case["M0"] and np.float32(smp) == np.float32(case["sampl_out"])  # grid.cpp:117
+        # bitwise the C-ABI plan's images
+        p = gr.Plan(n_ang=g.n_ang, n_det=g.n_det, center=g.center, geom=g.geom, angles=g.angles, filter=g.filter,
+                    C_=g.C, sampl=g.sampl, MaxPixSiz=g.MaxPixSiz, R=g.R, X0=g.X0, Y0=g.Y0, ltbl=g.ltbl)
+        ref_plan = p.recon(torch.from_numpy(np.stack([a, b])).cuda()).cpu().numpy()
+        p.close()
+        assert np.array_equal(out, ref_plan), i
+        d = disk(imgsiz)
+        for img, r in zip(out, arr[f"case{i:02d}_out"]):
+            assert O.rel_l2(img[d], r[d]) < 1e-5 and O.rel_l2(img, r) < 1e-4, i
+
+
+def test_dropin_cfg1_and_odd_slice_count(golden):
+    """cfg1 (256 detectors, 180 angles, Shepp-Logan) through the drop-in with
+    the caller's half padding done on the host, three slices: the last pair is
+    the third slice with itself (Multitomo_unit.txt:834-839 pairs a single
+    slice with itself)."""
+    meta, arr = golden
+    m = meta["cfg1"]
+    s0, s1 = arr["cfg1_in"]
+    rows = np.stack([O.pad_half(s0), O.pad_half(s1), O.pad_half(s0)])
+    g = O.Geometry(n_ang=m["n_ang"], n_det=m["n_det"], center=m["center"], filter=m["filter"]).padded()
+    out, imgsiz, _, rc, err = D.recon(0, rows, g.n_det, g.center, filter=g.filter, M0=m["M0"])
+    assert rc == 0 and err == "" and imgsiz == m["M0"]
+    off, n = m["crop_off"], m["crop_n"]
+    r1, r2 = arr["cfg1_out"]
+    for img, r in ((out[0], r1), (out[1], r2), (out[2], r1)):
+        e = O.rel_l2(img[off:off + n, off:off + n], r)
+        print(f"cfg1 through grid.h: rel-L2 vs reference {e:.2e}", flush=True)
+        assert e < 1e-5
+    # the lone slice rides with itself: its image equals the one it got beside
+    # another slice up to the rounding of the shared transform
+    assert O.rel_l2(out[2], out[0]) < 1e-6
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
+def test_dropin_against_reference_build():
+    """Angle list (geom 0), ROI offset and a hann filter, against the
+    reference's own do_recon compiled from its sources."""
+    rng = np.random.default_rng(41)
+    n_ang, n_det = 90, 200
+    ang = np.sort(rng.uniform(0, 180, n_ang)).astype(np.float32)
+    g = O.Geometry(n_ang=n_ang, n_det=n_det, center=98.6, geom=0, angles=ang, filter="hann", X0=0.05, Y0=-0.1)
+    a, b = rng.standard_normal((2, n_ang, n_det)).astype(np.float32)
+    M0 = O.sizes(g)["M0"]
+    out, _, _, rc, err = D.recon(1, np.stack([a, b]), n_det, g.center, geom=0, angles=ang, filter="hann",
+                                 X0=0.05, Y0=-0.1, M0=M0)
+    assert rc == 0 and err == ""
+    r1, r2 = O.ref_recon(g, a, b)
+    d = disk(M0)
+    for img, r in ((out[0], r1), (out[1], r2)):
+        assert O.rel_l2(img[d], r[d]) < 1e-5 and O.rel_l2(img, r) < 1e-4

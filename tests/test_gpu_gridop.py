@@ -30,6 +30,11 @@ CASES = {
     "roi-offset": dict(n_ang=120, n_det=128, center=64.0, X0=0.1, Y0=-0.05),
     "hann-C4": dict(n_ang=75, n_det=160, center=81.0, filter="hann", C_=4.0),
     "interp": dict(n_ang=90, n_det=128, center=63.5, interp=True),
+    # twelve angles within 15 degrees: the tiles along them take more than 64
+    # rows out to large slot indices, so the leading (one-pair-per-wave) slots
+    # of the sectors across the lines hold tiles no sample reaches (zero rows)
+    "clustered-angles": dict(n_ang=12, n_det=128, center=64.2, geom=0,
+                             angles=np.linspace(0.0, 15.0, 12).astype(np.float32)),
 }
 
 

@@ -69,7 +69,21 @@ def test_plan_tables_bit_exact(golden):
 
 
 def test_golden_small_cases(golden):
+    """Every golden case of the reference's gridrec: against its output (disk
+    1e-5, whole image 1e-4), and -- whole M0 x M0 images -- against the FP64
+    oracle, where the GPU must be at least as accurate as the reference
+    itself.  That is the claim behind the whole-image allowance: two FP32
+    implementations, each about e from the exact result, differ by about
+    sqrt(2) e, and on whole images e is ~1e-5 for the reference too (the
+    deapodization factor winv^2 amplifies FFT rounding in the corners).
+    Over the 58 images: the RMS of the GPU's errors is at most the
+    reference's, its largest error at most the reference's largest, and no
+    image's error exceeds 3x the reference's on the same image (each error is
+    dominated by a few corner pixels, so per-image ratios of two independent
+    roundings scatter; measured: RMS 6.6e-6 against 8.3e-6, the GPU more
+    accurate on 74% of the images, largest ratio 2.4)."""
     meta, arr = golden
+    worse, errs = [], []
     for i, case in enumerate(meta["cases"]):
         g = geometry_of(case, arr, i)
         p = plan_for(g)
@@ -77,9 +91,23 @@ def test_golden_small_cases(golden):
         out = gpu_pair(p, a, b)
         r1, r2 = arr[f"case{i:02d}_out"]
         (f1, d1), (f2, d2) = image_errors(out[0], r1), image_errors(out[1], r2)
-        print(f"case {i}: rel-L2 vs reference: disk {d1:.2e} {d2:.2e}, whole {f1:.2e} {f2:.2e}", flush=True)
+        x1, x2 = O.recon_f64(g, a, b)
+        gpu64 = (O.rel_l2(out[0], x1), O.rel_l2(out[1], x2))
+        ref64 = (O.rel_l2(r1, x1), O.rel_l2(r2, x2))
+        print(f"case {i}: rel-L2 vs reference: disk {d1:.2e} {d2:.2e}, whole {f1:.2e} {f2:.2e}; "
+              f"vs FP64: GPU {gpu64[0]:.2e} {gpu64[1]:.2e}, reference {ref64[0]:.2e} {ref64[1]:.2e}", flush=True)
         assert d1 < DISK_TOL and d2 < DISK_TOL and f1 < FULL_TOL and f2 < FULL_TOL, (i, d1, d2, f1, f2)
+        for k in range(2):
+            errs.append((gpu64[k], ref64[k]))
+            if gpu64[k] > 3.0 * ref64[k]:
+                worse.append((i, k, gpu64[k], ref64[k]))
         p.close()
+    g, r = np.array(errs).T
+    print(f"whole images vs FP64: RMS GPU {np.sqrt(np.mean(g ** 2)):.2e}, reference {np.sqrt(np.mean(r ** 2)):.2e}; "
+          f"max GPU {g.max():.2e}, reference {r.max():.2e}; GPU better on {np.mean(g <= r):.0%}", flush=True)
+    assert np.sqrt(np.mean(g ** 2)) <= np.sqrt(np.mean(r ** 2))
+    assert g.max() <= r.max()
+    assert not worse, worse
 
 
 def test_golden_cfg1_onchip_padding(golden):

@@ -168,3 +168,79 @@ def test_queue_with_angle_lists():
     disk = np.hypot(yy - M0 // 2, xx - M0 // 2) < 0.9 * M0 / 2
     assert O.rel_l2(o[0][disk], r1[disk]) < 1e-5 and O.rel_l2(o[1][disk], r2[disk]) < 1e-5
     q.close()
+
+
+@pytest.mark.parametrize("autotune", [False, True])
+def test_queue_one_size_class_uneven_angles(autotune):
+    """One size class whose dataset with the most slices has the fewest angles:
+    the class is tuned on that dataset (up to 8 pairs per launch), and the
+    others -- one pair with 20 times the angles -- must still get a workspace
+    that holds their spectra and grids; their images equal their plain plans."""
+    specs = [(16, 30), (2, 600), (1, 450), (3, 200)]  # (slices, angles), width 128
+    sets = []
+    for d, (ns, na) in enumerate(specs):
+        c = 64.0 + 0.7 * d
+        sino = torch.empty((ns, na, 128), dtype=torch.float32, device="cuda")
+        gr.phantom_sinograms(sino, seed=300 + d, n_ang=na, n_det=128, centers=[c] * ns, n_ell=6, photons=1e4)
+        kw = dict(n_ang=na, n_det=128, center=c, pad="half", crop=(64, 128))
+        out = torch.full((ns, 128, 128), float("nan"), dtype=torch.float32, device="cuda")
+        sets.append((kw, sino, out))
+    ref = plain_results(sets)
+    q = gr.Queue(sets, n_streams=2, autotune=autotune)
+    torch.cuda.synchronize()
+    assert q.info["size_classes"] == 1
+    for d, (ns, _) in enumerate(specs):
+        assert q.settings(d)["max_pairs"] <= max(1, (ns + 1) // 2), (d, q.settings(d))
+    for rep in range(2):
+        if rep:
+            for _, _, out in sets:
+                out.fill_(float("nan"))
+            q.run()
+            torch.cuda.synchronize()
+        for d, (_, _, out) in enumerate(sets):
+            assert same_as_plain(q, d, out, ref[d]), (rep, d)
+    q.close()
+
+
+def test_queue_at_production_widths():
+    """The uneven queue at the sizes of the production batch: detector widths
+    512 to 2560 (all five, two of them not powers of two), 180 to 1800 angles,
+    1 to 4 slices per dataset, centres off the middle.  Every dataset's images
+    are those of its plain plan (bitwise at the default radix), and three of
+    them match the FP32 oracle on the window to 1e-5."""
+    specs = [(512, 180, 3), (1024, 1800, 1), (1536, 700, 2), (2048, 1500, 4), (2560, 400, 2),
+             (512, 1333, 4), (2048, 250, 1), (1024, 960, 2)]
+    rng = np.random.default_rng(31)
+    sets = []
+    for d, (n, na, ns) in enumerate(specs):
+        c = float(np.float32(n / 2 + rng.uniform(-5, 5)))
+        sino = torch.empty((ns, na, n), dtype=torch.float32, device="cuda")
+        gr.phantom_sinograms(sino, seed=500 + d, n_ang=na, n_det=n, centers=[c] * ns, n_ell=20, photons=1e4)
+        kw = dict(n_ang=na, n_det=n, center=c, pad="half")
+        M0 = (1 << (2 * n - 1).bit_length()) - 1
+        cn = int(round(n * M0 / (2 * n)))
+        kw["crop"] = ((M0 - cn + 1) // 2, cn)
+        out = torch.full((ns, cn, cn), float("nan"), dtype=torch.float32, device="cuda")
+        sets.append((kw, sino, out))
+    ref = plain_results(sets)
+    q = gr.Queue(sets, n_streams=4, autotune=True)
+    q.run()
+    torch.cuda.synchronize()
+    print("queue info", q.info, flush=True)
+    # size classes (pdim, M, window): 1536 and 2048 share one -- both pdim =
+    # M = 4096, and the 1536 window round(1536 * 4095 / 3072) = 2048
+    assert q.info["size_classes"] == 4
+    for d, (_, _, out) in enumerate(sets):
+        assert same_as_plain(q, d, out, ref[d]), d
+    for d in (0, 7, 2):
+        kw, sino, out = sets[d]
+        g = O.Geometry(n_ang=kw["n_ang"], n_det=kw["n_det"], center=kw["center"]).padded()
+        s = sino.cpu().numpy()
+        r1, r2 = O.recon_f32(g, O.pad_half(s[0]), O.pad_half(s[1]))
+        off, cn = kw["crop"]
+        o = out.cpu().numpy()
+        for img, r in ((o[0], r1), (o[1], r2)):
+            e = O.rel_l2(img, r[off:off + cn, off:off + cn])
+            print(f"dataset {d} ({kw['n_det']} x {kw['n_ang']}): rel-L2 vs oracle {e:.2e}", flush=True)
+            assert e <= 1e-5, (d, e)
+    q.close()

@@ -7,6 +7,7 @@ closed form fails here.
 import math
 
 import numpy as np
+import pytest
 
 from projects_amd import phantom
 
@@ -63,3 +64,12 @@ def test_texture_amplitude_statistics():
     assert abs(np.sqrt(np.sum(T.amp.astype(np.float64) ** 2)) - 0.5) < 0.02
     f = np.hypot(T.fx, T.fy)
     assert f.max() <= 10.0 and abs(np.mean(f <= 5.0) - 0.25) < 0.02  # uniform over the disk
+
+
+def test_phantom_image_torch_matches_numpy():
+    """The torch evaluation of the phantom image (used for large benchmark
+    images) is the numpy one up to the last bits of cos."""
+    torch = pytest.importorskip("torch")
+    a = phantom.phantom_image(5, 3, 20, 128, n_tex=64, tex_amp=0.15, tex_fmax=24.0)
+    b = phantom.phantom_image(5, 3, 20, 128, n_tex=64, tex_amp=0.15, tex_fmax=24.0, device="cpu")
+    assert b.dtype == torch.float64 and np.abs(a - b.numpy()).max() < 1e-12
