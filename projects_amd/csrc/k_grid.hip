@@ -455,6 +455,13 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
     const unsigned n = O.tile_n[ti] & kGopRows;
     float4 acc[1] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
     float2 part[1] = {make_float2(0.0f, 0.0f)};
+    if (n == 0) {
+        // a slot no sample reaches: no entry may be read (n - 1 would wrap and
+        // the clamped prefetch below would run past the tile's entries); the
+        // cells are still stored as zeros, which the row transform reads
+        store_cells<ACCUM, 1>(A, pair, 1, a, b, acc);
+        return;
+    }
     auto entry = [&](unsigned k) {
         const uint2 t = gop_load(e + 64ul * min(k, n - 1));
         return k < n ? t : make_uint2(0u, 0u);
