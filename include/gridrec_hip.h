@@ -195,6 +195,22 @@ int gr_recon_device(gr_plan *plan, const float *sino, long sino_slice_stride,
                     long sino_row_stride, float *out, long out_slice_stride,
                     long out_row_stride, int n_slices, void *stream);
 
+/* gr_recon_device without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, layouts and images (bitwise) as gr_recon_device; the kernels'
+ * scratch (spectra, gridded and row-transformed planes) is one static
+ * per-GPU array of gr_noalloc_scratch_bytes() bytes (128 MiB) instead of the
+ * plan's workspace.  GR_EINVAL when one slice pair's scratch exceeds it
+ * (e.g. a half-padded 1024-detector geometry beyond about 600 angles).  Calls
+ * that overlap on one GPU race on that array: serialise them.  A plan whose
+ * gather operator is not built yet evaluates the geometry per launch here;
+ * no per-kernel timing is recorded. */
+int gr_recon_device_noalloc(gr_plan *plan, const float *sino, long sino_slice_stride,
+                            long sino_row_stride, float *out, long out_slice_stride,
+                            long out_row_stride, int n_slices, void *stream);
+size_t gr_noalloc_scratch_bytes(void);
+
 /* do_recon from detector counts: the preprocessing the callers run on the
  * host before gridrec, fused into the row load of the same kernels.
  *   GR_PRE_FLATDARK:     v = 1 - (p - d) / (w1/2 + w2/2 - d)   (tomo_recon.cpp:1877)

@@ -97,6 +97,9 @@ _SIGS = {
                                   C.c_int, C.c_void_p]),
     "gr_recon_device_raw": (C.c_int, [C.c_void_p, C.POINTER(GrPreproc), C.c_void_p, C.c_long, C.c_long, C.c_int,
                                       C.c_void_p]),
+    "gr_recon_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.c_void_p, C.c_long,
+                                          C.c_long, C.c_int, C.c_void_p]),
+    "gr_noalloc_scratch_bytes": (C.c_size_t, []),
     "gr_center_entropy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_long, C.c_long, C.POINTER(C.c_float), C.c_int,
                                     C.c_float, C.c_float, C.c_int, C.POINTER(C.c_float), C.c_void_p]),
     "gr_ring_correct": (C.c_int, [C.c_void_p, C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
@@ -281,6 +284,23 @@ class Plan:
                                    C.c_void_p(out.data_ptr()), out.stride(0), out.stride(1), ns,
                                    C.c_void_p(_stream_handle(stream)))
         _check(rc, "gr_recon_device")
+        return out
+
+    def recon_noalloc(self, sino, out, stream=None):
+        """recon() through gr_recon_device_noalloc: the kernels' scratch is the
+        library's static per-GPU array, nothing is allocated (out is required).
+        Calls that overlap on one GPU race on that array."""
+        import torch
+
+        assert sino.is_cuda and sino.dtype == torch.float32 and sino.dim() == 3
+        assert sino.shape[1] == self.n_ang and sino.shape[2] >= self.n_det and sino.stride(2) == 1
+        ns, on = sino.shape[0], self.out_n
+        assert out.is_cuda and out.dtype == torch.float32 and out.shape[0] == ns and out.stride(2) == 1
+        assert out.shape[1] >= on and out.shape[2] >= on
+        rc = lib().gr_recon_device_noalloc(self._h, C.c_void_p(sino.data_ptr()), sino.stride(0), sino.stride(1),
+                                           C.c_void_p(out.data_ptr()), out.stride(0), out.stride(1), ns,
+                                           C.c_void_p(_stream_handle(stream)))
+        _check(rc, "gr_recon_device_noalloc")
         return out
 
     def recon_raw(self, raw, dark, white1, white2=None, mode: str = "flatdark", xoffset: float = 0.0, out=None,
