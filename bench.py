@@ -296,6 +296,15 @@ def bench_batched(a, gr, O, torch):
                        "rel_l2_vs_reference_restatement": float(f"{max(O.rel_l2(o[0], r1[c, c]), O.rel_l2(o[1], r2[c, c])):.3e}"),
                        "rel_l2_vs_fp64": float(f"{max(O.rel_l2(o[0], d1[c, c]), O.rel_l2(o[1], d2[c, c])):.3e}")}
                 break
+    # whole-pipeline algorithmic bytes of the batch (the plain line's formula
+    # per slice pair: 8 n_ang n_det + 16 n_ang pdim + 32 M^2 + 8 n_out^2, with
+    # pdim = M = the power of two >= the half-padded width)
+    batch_bytes = 0
+    for kw, sino, out in sets:
+        pd = 1 << (2 * kw["n_det"] - 1).bit_length()
+        pairs = (sino.shape[0] + 1) // 2
+        batch_bytes += pairs * (8 * kw["n_ang"] * kw["n_det"] + 16 * kw["n_ang"] * pd + 32 * pd * pd +
+                                8 * out.shape[1] * out.shape[1])
     info = q.info
     settings = {}
     for d, (kw, _, _) in enumerate(sets):
@@ -313,6 +322,8 @@ def bench_batched(a, gr, O, torch):
         "per_dataset_operator_build_s": round(op_build_s, 2),
         "queue_datasets_with_operator": info["datasets_with_operator"],
         "queue_operator_GB": round(info["operator_bytes"] / 1e9, 2),
+        "pipeline_alg_GBs": round(batch_bytes / (queue_ms * 1e-3) / 1e9, 1),
+        "per_dataset_plain_pipeline_alg_GBs": round(batch_bytes / (plain_ms * 1e-3) / 1e9, 1),
         "queue_ms_per_batch": round(queue_ms, 2), "per_dataset_plain_ms": round(plain_ms, 2),
         "per_dataset_plan_setup_s": round(setup_s, 2), "queue_build_s": round(build_s, 2),
         "bitwise_equal_to_per_dataset_plain": bool(same),

@@ -398,39 +398,103 @@ __global__ void __launch_bounds__(256) k_conv_in(const float *__restrict__ x, fl
     }
 }
 
-// Weight gradient of the 1 -> 32 layer: partial[block] = (dW1[co][t], db1[co]).
-__global__ void __launch_bounds__(320) k_conv_in_wgrad(const float *__restrict__ x, float scale,
+// Sum acc[9][8] (tap, channel of the thread's group of 8) and extra[8] over
+// the threads of the block that share a channel group (tid mod 4), in a fixed
+// order: lanes of a wave by a butterfly over lane bits 2..5, then the four
+// waves in order through LDS; the block's record gets W[ch][t] at
+// ch * 9 + t (parameter order) and the extras at 9 * CH + ch.
+__device__ __forceinline__ void block_sum_groups(float (&acc)[9][8], float (&extra)[8], float *red, float *out,
+                                                 bool with_extra) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, cg = tid & 3;
+#pragma unroll
+    for (int o = 4; o < 64; o <<= 1) {
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) acc[t][e] += __shfl_xor(acc[t][e], o);
+#pragma unroll
+        for (int e = 0; e < 8; e++) extra[e] += __shfl_xor(extra[e], o);
+    }
+    // lanes 0..3 of each wave hold the wave's sums of groups 0..3
+    if (lane < 4) {
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) red[(wave * 4 + cg) * 80 + t * 8 + e] = acc[t][e];
+#pragma unroll
+        for (int e = 0; e < 8; e++) red[(wave * 4 + cg) * 80 + 72 + e] = extra[e];
+    }
+    __syncthreads();
+    for (int i = tid; i < 4 * 80; i += blockDim.x) {  // i = group * 80 + slot
+        const int grp = i / 80, sl = i - grp * 80;
+        if (sl >= 72 && !with_extra) continue;
+        float v = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) v += red[(w * 4 + grp) * 80 + sl];
+        if (sl < 72) {
+            const int t = sl >> 3, ch = grp * 8 + (sl & 7);
+            out[ch * 9 + t] = v;
+        } else {
+            out[9 * CH + grp * 8 + (sl - 72)] = v;
+        }
+    }
+}
+
+// Weight gradient of the 1 -> 32 layer: partial[block] = (dW1[co][t], db1[co])
+// over the tiles blockIdx.x, blockIdx.x + gridDim.x, ...  Thread tid takes the
+// 8 channels (tid mod 4) * 8.. of the output pixels tid / 4 + 64 k of a tile:
+// one 16-byte load of dz per pixel (four threads read a pixel's 64-byte
+// record, a wave 16 whole records), all four issued before use, and the nine
+// taps of bf16(x/scale) from the tile staged with its halo in LDS; the sums
+// stay in registers across the block's tiles.
+__global__ void __launch_bounds__(256) k_conv_in_wgrad(const float *__restrict__ x, float scale,
                                                        const uint16_t *__restrict__ dz, float *__restrict__ part,
                                                        int H, int W, int ntiles, TileMap tm) {
     __shared__ float sx[HS * WS];
-    __shared__ float sdz[TH * TW * (CH + 1)];
-    const int tid = threadIdx.x;
-    float acc = 0.0f;
+    __shared__ float red[16 * 80];
+    const int tid = threadIdx.x, cg = tid & 3, ps = tid >> 2;
+    float acc[9][8], db[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        db[e] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) acc[t][e] = 0.0f;
+    }
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         int img, y0, x0;
         tm.of(tile, img, y0, x0);
         const long ibase = (long)img * H * W;
-        __syncthreads();
-        for (int i = tid; i < HS * WS; i += 320) {
+        uint4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int p = ps + 64 * k, r = p / TW, c = p - r * TW, gy = y0 + r, gx = x0 + c;
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (gy < H && gx < W) v[k] = *reinterpret_cast<const uint4 *>(dz + (ibase + (long)gy * W + gx) * CH + cg * 8);
+        }
+        __syncthreads();  // the previous tile's reads of sx are done
+        for (int i = tid; i < HS * WS; i += 256) {
             const int r = i / WS, c = i - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
             sx[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(x[ibase + (long)gy * W + gx] / scale) : 0.0f;
         }
-        for (int i = tid; i < TH * TW * CH; i += 320) {
-            const int pix = i / CH, co = i - pix * CH, r = pix / TW, c = pix - r * TW;
-            const int gy = y0 + r, gx = x0 + c;
-            sdz[pix * (CH + 1) + co] = (gy < H && gx < W) ? from_bf16(dz[(ibase + (long)gy * W + gx) * CH + co]) : 0.0f;
-        }
         __syncthreads();
-        if (tid < 9 * CH) {
-            const int co = tid / 9, t = tid - co * 9, ky = t / 3, kx = t - ky * 3;
-            for (int r = 0; r < TH; r++)
-                for (int c = 0; c < TW; c++) acc += sdz[(r * TW + c) * (CH + 1) + co] * sx[(r + ky) * WS + c + kx];
-        } else if (tid < IN_REC) {
-            const int co = tid - 9 * CH;
-            for (int pix = 0; pix < TH * TW; pix++) acc += sdz[pix * (CH + 1) + co];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int p = ps + 64 * k, r = p / TW, c = p - r * TW;
+            const uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            float d[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                d[e] = from_bf16((uint16_t)(w4[e >> 1] >> (16 * (e & 1))));
+                db[e] += d[e];
+            }
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const float xv = sx[(r + t / 3) * WS + c + t % 3];
+#pragma unroll
+                for (int e = 0; e < 8; e++) acc[t][e] = __builtin_fmaf(d[e], xv, acc[t][e]);
+            }
         }
     }
-    if (tid < IN_REC) part[(long)blockIdx.x * IN_REC + tid] = acc;
+    block_sum_groups(acc, db, red, part + (long)blockIdx.x * IN_REC, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -730,50 +794,107 @@ __global__ void __launch_bounds__(256) k_conv_out_dgrad(const float *__restrict_
     }
 }
 
-// Weight gradient of the 32 -> 1 layer: partial[block] = (dW5[ci][t], db5).
-__global__ void __launch_bounds__(320) k_conv_out_wgrad(const float *__restrict__ dy, const uint16_t *__restrict__ a4,
+// Weight gradient of the 32 -> 1 layer: partial[block] = (dW5[ci][t], db5)
+// over the tiles blockIdx.x, blockIdx.x + gridDim.x, ...  dW5[ci][t] = sum_p
+// dy[p] a4[p + off_t][ci] is taken as a sum over the input pixels q of the
+// tile with its halo, dy[q - off_t] the output pixel each tap reaches: thread
+// tid takes the 8 channels (tid mod 4) * 8.. of the halo pixels tid / 4 +
+// 64 k (one 16-byte load per pixel, a wave 16 whole records, all six issued
+// before use) and dy of the tile from LDS; sums in registers across tiles.
+__global__ void __launch_bounds__(256) k_conv_out_wgrad(const float *__restrict__ dy, const uint16_t *__restrict__ a4,
                                                         float *__restrict__ part, int H, int W, int ntiles,
                                                         TileMap tm) {
     __shared__ float sdy[TH * TW];
-    __shared__ float sa[HS * WS * (CH + 1)];
-    const int tid = threadIdx.x;
-    float acc = 0.0f;
+    __shared__ float red[16 * 80];
+    constexpr int NQ = (HS * WS + 63) / 64;  // halo pixels per thread: 6
+    const int tid = threadIdx.x, cg = tid & 3, ps = tid >> 2;
+    float acc[9][8], db[8], dsum = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        db[e] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 9; t++) acc[t][e] = 0.0f;
+    }
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         int img, y0, x0;
         tm.of(tile, img, y0, x0);
         const long ibase = (long)img * H * W;
-        __syncthreads();
-        for (int i = tid; i < TH * TW; i += 320) {
-            const int r = i / TW, c = i - r * TW, gy = y0 + r, gx = x0 + c;
-            sdy[i] = (gy < H && gx < W) ? dy[ibase + (long)gy * W + gx] : 0.0f;
+        uint4 v[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            const int q = ps + 64 * k, r = q / WS, c = q - r * WS, gy = y0 + r - 1, gx = x0 + c - 1;
+            v[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (q < HS * WS && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                v[k] = *reinterpret_cast<const uint4 *>(a4 + (ibase + (long)gy * W + gx) * CH + cg * 8);
         }
-        for (int i = tid; i < HS * WS * CH; i += 320) {
-            const int pix = i / CH, ci = i - pix * CH, r = pix / WS, c = pix - r * WS;
-            const int gy = y0 + r - 1, gx = x0 + c - 1;
-            sa[pix * (CH + 1) + ci] =
-                (gy >= 0 && gy < H && gx >= 0 && gx < W) ? from_bf16(a4[(ibase + (long)gy * W + gx) * CH + ci]) : 0.0f;
+        __syncthreads();  // the previous tile's reads of sdy are done
+        {
+            const int r = tid / TW, c = tid - r * TW, gy = y0 + r, gx = x0 + c;
+            const float d = (gy < H && gx < W) ? dy[ibase + (long)gy * W + gx] : 0.0f;
+            sdy[tid] = d;
+            dsum += d;  // db5: thread tid takes pixel tid of every tile
         }
         __syncthreads();
-        if (tid < 9 * CH) {
-            const int ci = tid / 9, t = tid - ci * 9, ky = t / 3, kx = t - ky * 3;
-            for (int r = 0; r < TH; r++)
-                for (int c = 0; c < TW; c++) acc += sdy[r * TW + c] * sa[((r + ky) * WS + c + kx) * (CH + 1) + ci];
-        } else if (tid < OUT_REC) {
-            for (int i = 0; i < TH * TW; i++) acc += sdy[i];
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+            const int q = ps + 64 * k, qr = q / WS, qc = q - qr * WS;
+            const uint32_t w4[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            float a[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) a[e] = from_bf16((uint16_t)(w4[e >> 1] >> (16 * (e & 1))));
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int pr = qr - t / 3, pc = qc - t % 3;  // output pixel of tap t
+                const float d = (q < HS * WS && pr >= 0 && pr < TH && pc >= 0 && pc < TW) ? sdy[pr * TW + pc] : 0.0f;
+#pragma unroll
+                for (int e = 0; e < 8; e++) acc[t][e] = __builtin_fmaf(d, a[e], acc[t][e]);
+            }
         }
     }
-    if (tid < OUT_REC) part[(long)blockIdx.x * OUT_REC + tid] = acc;
+    float *out = part + (long)blockIdx.x * OUT_REC;
+    block_sum_groups(acc, db, red, out, false);
+    // db5: the threads' pixel sums in thread order
+    __syncthreads();
+    red[tid] = dsum;
+    __syncthreads();
+    if (tid == 0) {
+        float s5 = 0.0f;
+        for (int i = 0; i < 256; i++) s5 += red[i];
+        out[9 * CH] = s5;
+    }
 }
 
 // Sum of nparts partial records in block order (double) into the gradient
 // vector.  kind 0: 32->32 record (t,co,ci)+db -> W[co][ci][t], b; kind 1 and 2
 // records are already in parameter order.
-__global__ void k_reduce(const float *__restrict__ part, int nparts, int rec, int kind, float *__restrict__ gw,
-                         float *__restrict__ gb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= rec) return;
+// A block of 1024 threads takes 64 consecutive elements: wave w sums the
+// records w, w + 16, ... for its lane's element (coalesced 256-byte rows, eight
+// loads in flight per batch), then lane e of wave 0 adds the 16 wave sums in
+// wave order -- a fixed order, so results are bitwise repeatable.
+constexpr int RED_WAVES = 16;
+__global__ void __launch_bounds__(1024) k_reduce(const float *__restrict__ part, int nparts, int rec, int kind,
+                                                 float *__restrict__ gw, float *__restrict__ gb) {
+    __shared__ double sw[RED_WAVES][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (int k = 0; k < nparts; k++) s += (double)part[(long)k * rec + i];
+    if (i < rec) {
+        for (int k0 = wave; k0 < nparts; k0 += 8 * RED_WAVES) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int k = k0 + u * RED_WAVES;
+                v[u] = k < nparts ? part[(long)k * rec + i] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += (double)v[u];
+        }
+    }
+    sw[wave][lane] = s;
+    __syncthreads();
+    if (wave != 0 || i >= rec) return;
+    s = 0.0;
+    for (int w = 0; w < RED_WAVES; w++) s += sw[w][lane];
     const float v = (float)s;
     if (kind == 0) {
         if (i < 9 * CH * CH) {
@@ -791,14 +912,34 @@ __global__ void k_reduce(const float *__restrict__ part, int nparts, int rec, in
 
 // Sum of per-block partials in order, times f: out[grp*stride + comp] =
 // f * sum_i part[(grp*per + i)*stride + comp].
-__global__ void k_sum_groups(const float *__restrict__ part, int groups, int per, int stride, double f,
-                             float *__restrict__ out) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// One block of 256 threads per output: thread t adds the partials t, t + 256,
+// ... in order (double, eight loads in flight per batch), then the 256 sums
+// are added by a fixed pairwise tree -- bitwise repeatable.
+__global__ void __launch_bounds__(256) k_sum_groups(const float *__restrict__ part, int groups, int per, int stride,
+                                                    double f, float *__restrict__ out) {
+    __shared__ double sd[256];
+    const int k = blockIdx.x, tid = threadIdx.x;
     if (k >= groups * stride) return;
     const int grp = k / stride, comp = k - grp * stride;
+    const float *src = part + (long)grp * per * stride + comp;
     double s = 0.0;
-    for (int i = 0; i < per; i++) s += (double)part[((long)grp * per + i) * stride + comp];
-    out[k] = (float)(s * f);
+    for (int i0 = tid; i0 < per; i0 += 8 * 256) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u * 256;
+            v[u] = i < per ? src[(long)i * stride] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += (double)v[u];
+    }
+    sd[tid] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) sd[tid] += sd[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) out[k] = (float)(sd[0] * f);
 }
 
 // Packed bf16 weights of the three 32->32 layers: forward [t][co][ci] and
@@ -902,7 +1043,7 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     hipLaunchKernelGGL(k_conv_out, dim3(ntiles), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W, tm);
     if (slice_stats) {
         const int per = tm.tiles_x * tm.tiles_y;
-        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, per, 2, 1.0,
+        hipLaunchKernelGGL(k_sum_groups, dim3(2 * n), dim3(256), 0, s, stats, n, per, 2, 1.0,
                            slice_stats);
     }
     CNN_HIP(hipGetLastError());
@@ -929,7 +1070,7 @@ int gr_cnn_infer(const float *params, const unsigned short *packed, const float 
     hipLaunchKernelGGL(k_cnn_fused, dim3((unsigned)nblocks), dim3(FH * FW), FUSED_LDS, s, x, scale, params,
                        reinterpret_cast<const uint16_t *>(packed), y, stats, H, W, tx, ty);
     if (slice_stats)
-        hipLaunchKernelGGL(k_sum_groups, dim3((2 * n + 255) / 256), dim3(256), 0, s, stats, n, tx * ty, 2, 1.0,
+        hipLaunchKernelGGL(k_sum_groups, dim3(2 * n), dim3(256), 0, s, stats, n, tx * ty, 2, 1.0,
                            slice_stats);
     CNN_HIP(hipGetLastError());
     return GR_OK;
@@ -965,10 +1106,10 @@ int gr_cnn_backward(const float *params, const unsigned short *packed, const flo
     const int nloss = (int)std::min<long>(1024, (npx + 255) / 256);
 
     hipLaunchKernelGGL(k_loss, dim3(nloss), dim3(256), 0, s, yn, target, scale, dy, lpart, npx);
-    if (loss) hipLaunchKernelGGL(k_sum_groups, dim3(1), dim3(64), 0, s, lpart, 1, nloss, 1, 1.0 / (double)npx, loss);
+    if (loss) hipLaunchKernelGGL(k_sum_groups, dim3(1), dim3(256), 0, s, lpart, 1, nloss, 1, 1.0 / (double)npx, loss);
     // layer 5
-    hipLaunchKernelGGL(k_conv_out_wgrad, dim3(nwg), dim3(320), 0, s, dy, a4, part, H, W, ntiles, tm);
-    hipLaunchKernelGGL(k_reduce, dim3((OUT_REC + 255) / 256), dim3(256), 0, s, part, nwg, OUT_REC, 1,
+    hipLaunchKernelGGL(k_conv_out_wgrad, dim3(nwg), dim3(256), 0, s, dy, a4, part, H, W, ntiles, tm);
+    hipLaunchKernelGGL(k_reduce, dim3((OUT_REC + 63) / 64), dim3(64 * RED_WAVES), 0, s, part, nwg, OUT_REC, 1,
                        grads + OFF_W5, grads + OFF_B5);
     hipLaunchKernelGGL(k_conv_out_dgrad, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s, dy, a4, params, dzA,
                        n, H, W);
@@ -980,7 +1121,7 @@ int gr_cnn_backward(const float *params, const unsigned short *packed, const flo
     for (int l = 2; l >= 0; l--) {  // layer l+2: input acts[l], output gradient dz
         const uint16_t *ain = reinterpret_cast<const uint16_t *>(acts[l]);
         hipLaunchKernelGGL(k_conv32_wgrad, dim3(nwg), dim3(256), WGRAD_LDS, s, ain, dz, part, H, W, ntiles, tm);
-        hipLaunchKernelGGL(k_reduce, dim3((WG_REC + 255) / 256), dim3(256), 0, s, part, nwg, WG_REC, 0,
+        hipLaunchKernelGGL(k_reduce, dim3((WG_REC + 63) / 64), dim3(64 * RED_WAVES), 0, s, part, nwg, WG_REC, 0,
                            grads + woff[l], grads + boff[l]);
         hipLaunchKernelGGL(k_conv32<1>, dim3(ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS), dim3(256), 0, s,
                            dz, pk + (long)l * PACK_LAYER + 9 * CH * CH, (const float *)nullptr, ain, dzn, H, W,
@@ -988,8 +1129,8 @@ int gr_cnn_backward(const float *params, const unsigned short *packed, const flo
         std::swap(dz, dzn);
     }
     // layer 1
-    hipLaunchKernelGGL(k_conv_in_wgrad, dim3(nwg), dim3(320), 0, s, x, scale, dz, part, H, W, ntiles, tm);
-    hipLaunchKernelGGL(k_reduce, dim3((IN_REC + 255) / 256), dim3(256), 0, s, part, nwg, IN_REC, 1,
+    hipLaunchKernelGGL(k_conv_in_wgrad, dim3(nwg), dim3(256), 0, s, x, scale, dz, part, H, W, ntiles, tm);
+    hipLaunchKernelGGL(k_reduce, dim3((IN_REC + 63) / 64), dim3(64 * RED_WAVES), 0, s, part, nwg, IN_REC, 1,
                        grads + OFF_W1, grads + OFF_B1);
     CNN_HIP(hipGetLastError());
     return GR_OK;
