@@ -255,7 +255,7 @@ void free_plan(gr_plan *pl) {
     if (!pl) return;
     free_staging(pl);
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
-                    pl->d_aord, pl->d_ang, pl->d_gop, pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl};
+                    pl->d_aord, pl->d_ang, pl->d_gop, pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl, pl->d_rowtiles};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -532,6 +532,16 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         pl->sector_len = slots / kGridSectors;
         pl->ntiles = slots;
         pl->dev_bytes += sizeof(unsigned) * (size_t)pl->ntiles;
+        // the row transform's active-tile interval of every tile row
+        CK(dev_alloc(pl, &pl->d_rowtiles, (size_t)nt));
+        hipStream_t rs;
+        CK(plan_stream(pl, &rs));
+        hipError_t e = gr_launch_row_tiles(g, pl->d_rowtiles, rs);
+        if (e == hipSuccess) e = hipStreamSynchronize(rs);
+        if (e != hipSuccess) {
+            free_plan(pl);
+            return hip_fail(e, "row tile intervals");
+        }
     }
     // workspace
     const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;

@@ -51,16 +51,24 @@ __host__ __device__ __forceinline__ void gr_sample_uv(int j, float C, float S, c
 
 // If sample j of the angle with table values (C, S) deposits into cell
 // (qa, qb), return true and its weight.  qa, qb are exact small integers.
-__host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float qa, float qb,
-                                                const GrGatherGeom &g, const float *wtbl,
-                                                float &w) {
+// gr_tap's table indices: the weight is wtbl[iu] * wtbl[iv].
+__host__ __device__ __forceinline__ bool gr_tap_index(int j, float C, float S, float qa, float qb,
+                                                      const GrGatherGeom &g, int &iu, int &iv) {
     float U, V;
     gr_sample_uv(j, C, S, g, U, V);
     if (!(U - g.L2 <= qa && qa <= U + g.L2 && V - g.L2 <= qb && qb <= V + g.L2)) return false;
-    int iu = gr_nn_index(fabsf(U - qa) * g.tblspcg);
-    int iv = gr_nn_index(fabsf(V - qb) * g.tblspcg);
+    iu = gr_nn_index(fabsf(U - qa) * g.tblspcg);
+    iv = gr_nn_index(fabsf(V - qb) * g.tblspcg);
     iu = iu > g.ltbl ? g.ltbl : iu;
     iv = iv > g.ltbl ? g.ltbl : iv;
+    return true;
+}
+
+__host__ __device__ __forceinline__ bool gr_tap(int j, float C, float S, float qa, float qb,
+                                                const GrGatherGeom &g, const float *wtbl,
+                                                float &w) {
+    int iu, iv;
+    if (!gr_tap_index(j, C, S, qa, qb, g, iu, iv)) return false;
     w = wtbl[iu] * wtbl[iv];
     return true;
 }

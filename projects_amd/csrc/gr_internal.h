@@ -57,6 +57,9 @@ struct GrDevTables {
     const uint2 *gop = nullptr;
     const unsigned *gop_n = nullptr, *gop_off = nullptr;
     int gop_heavy = 0;  // leading slots of every sector gridded one pair per wave
+    // per tile row (M / 8 rows) the interval [lo, hi] of tiles that can hold
+    // a nonzero cell (gr_active_tile_range), evaluated once on the device
+    const int2 *row_tiles = nullptr;
 };
 
 // ---- kernel launchers (one per .hip file) ----------------------------------
@@ -119,6 +122,10 @@ hipError_t gr_launch_gop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, 
 
 // K3: FFT (-1) of each row of Ht (over a), output rows of the window only:
 // Z[pairs][M][out_n].
+// The active-tile interval of every tile row of the M x M grid into
+// out[M / 8] (gr_active_tile_range on the device, the very arithmetic of the
+// gridding kernels' own test).
+hipError_t gr_launch_row_tiles(const GrGeom &g, int2 *out, hipStream_t s);
 hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
                                int pairs, hipStream_t s);
 

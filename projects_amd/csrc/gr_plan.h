@@ -19,6 +19,7 @@ struct gr_plan {
     int *d_aord = nullptr;
     float4 *d_ang = nullptr;
     unsigned *d_tiles = nullptr;
+    int2 *d_rowtiles = nullptr;  // active-tile interval per tile row (K3)
     int ntiles = 0;      // slots: kGridSectors * sector_len
     int sector_len = 0;
     // K2's precomputed gather operator (GR_GRIDOP_*): entries, taps per tile,
@@ -75,6 +76,7 @@ struct gr_plan {
         t.tiles = d_tiles;
         t.ntiles = ntiles;
         t.sector_len = sector_len;
+        t.row_tiles = d_rowtiles;
         if (gop_state == 1) {
             t.gop = d_gop;
             t.gop_n = d_gop_n;

@@ -256,19 +256,38 @@ __global__ void __launch_bounds__(256) k_conv32_wgrad(const uint16_t *__restrict
     for (int i = 0; i < 9; i++) acc[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     float db[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int ch = tid & 3;  // the channel chunk this thread stages (256 % 4 == 0)
-
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // The next tile's records are loaded into registers while this tile's are
+    // added (its loads are in flight during the MFMAs): DZ_N pieces of dz and
+    // A_N of a per thread.
+    constexpr int DZ_N = TH * TW * 4 / 256, A_N = (HS * WS * 4 + 255) / 256;
+    uint4 pdz[DZ_N], pa[A_N];
+    auto fetch = [&](int tile) {
         int img, y0, x0;
         tm.of(tile, img, y0, x0);
         const long ibase = (long)img * H * W;
+#pragma unroll
+        for (int k = 0; k < DZ_N; k++) {
+            const int pix = (tid + 256 * k) >> 2, r = pix / TW, c = pix - r * TW, gy = y0 + r, gx = x0 + c;
+            pdz[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (gy < H && gx < W) pdz[k] = *reinterpret_cast<const uint4 *>(dz + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+        }
+#pragma unroll
+        for (int k = 0; k < A_N; k++) {
+            const int i = tid + 256 * k, pix = i >> 2, r = pix / WS, cc = pix - r * WS;
+            const int gy = y0 + r - 1, gx = x0 + cc - 1;
+            pa[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (i < HS * WS * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                pa[k] = *reinterpret_cast<const uint4 *>(a + ((ibase + (long)gy * W + gx) * CH + ch * 8));
+        }
+    };
+    if (blockIdx.x < ntiles) fetch(blockIdx.x);
+
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         __syncthreads();  // previous tile's reads done
-        for (int i = tid; i < TH * TW * 4; i += 256) {
-            const int pix = i >> 2;
-            const int r = pix / TW, c = pix - r * TW;
-            const int gy = y0 + r, gx = x0 + c;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy < H && gx < W) v = *reinterpret_cast<const uint4 *>(dz + ((ibase + (long)gy * W + gx) * CH + ch * 8));
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < DZ_N; k++) {
+            const int pix = (tid + 256 * k) >> 2;
+            const uint32_t w[4] = {pdz[k].x, pdz[k].y, pdz[k].z, pdz[k].w};
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const uint16_t h = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
@@ -276,14 +295,12 @@ __global__ void __launch_bounds__(256) k_conv32_wgrad(const uint16_t *__restrict
                 db[e] += from_bf16(h);
             }
         }
-        for (int i = tid; i < HS * WS * 4; i += 256) {
-            const int pix = i >> 2;
+#pragma unroll
+        for (int k = 0; k < A_N; k++) {
+            const int i = tid + 256 * k, pix = i >> 2;
+            if (i >= HS * WS * 4) break;
             const int r = pix / WS, cc = pix - r * WS;
-            const int gy = y0 + r - 1, gx = x0 + cc - 1;
-            uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *reinterpret_cast<const uint4 *>(a + ((ibase + (long)gy * W + gx) * CH + ch * 8));
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            const uint32_t w[4] = {pa[k].x, pa[k].y, pa[k].z, pa[k].w};
 #pragma unroll
             for (int kx = 0; kx < 3; kx++) {
                 const int c = cc - kx;  // sa[kx] column c holds input column x0 + c + kx - 1
@@ -293,6 +310,7 @@ __global__ void __launch_bounds__(256) k_conv32_wgrad(const uint16_t *__restrict
                     sa[(kx * CH + ch * 8 + e) * SA_CI + r * TW + c] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
             }
         }
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
         __syncthreads();
 #pragma unroll 1
         for (int r = 0; r < TH; r++) {

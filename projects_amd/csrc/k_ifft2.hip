@@ -27,9 +27,9 @@ struct RowsArgs {
     const float2 *ht;
     float2 *z;
     const float2 *tw;
+    const int2 *row_tiles;  // per tile row the written tiles [lo, hi]; the others are zeros
     int M, radix;
     long M02, out_off, out_n;
-    float rmax;  // tiles beyond it were not written by the gridding step: zeros
 };
 
 // Output hook of the row transform: FFT output i lands in window column
@@ -60,8 +60,8 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
     const int M = A.M;
     float2 *row = lds + (long)grp * M;
     const float2 *src = A.ht + ((long)p * M + b) * M;
-    int lo, hi;
-    gr_active_tile_range(b >> 3, M >> 3, (float)(M >> 1), A.rmax, lo, hi);
+    const int2 lh = A.row_tiles[b >> 3];
+    const int lo = lh.x, hi = lh.y;
     float2 v[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
@@ -234,7 +234,22 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
 
 inline int maxb_of(long N, int nthr) { return (int)((N / 4 + nthr - 1) / nthr); }
 
+__global__ void k_row_tiles(int2 *out, int ntr, float M2, float rmax) {
+    const int ty = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ty >= ntr) return;
+    int lo, hi;
+    gr_active_tile_range(ty, ntr, M2, rmax, lo, hi);
+    out[ty] = make_int2(lo, hi);
+}
+
 }  // namespace
+
+hipError_t gr_launch_row_tiles(const GrGeom &g, int2 *out, hipStream_t s) {
+    const int ntr = (int)(g.M >> 3);
+    hipLaunchKernelGGL(k_row_tiles, dim3((ntr + 255) / 256), dim3(256), 0, s, out, ntr, (float)(g.M >> 1),
+                       gr_grid_rmax(g));
+    return hipGetLastError();
+}
 
 // Rows of Ht per workgroup of the row transform: one (at M = 4096 on MI355X
 // 0.57 ms per launch of 8 pairs, 0.59 with 2 side by side, 0.67 with 4).
@@ -242,7 +257,7 @@ constexpr int kRowsPerBlock = 1;
 
 hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const float2 *ht, float2 *z,
                                int pairs, hipStream_t s) {
-    RowsArgs A{ht, z, t.tw_grid, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n, gr_grid_rmax(g)};
+    RowsArgs A{ht, z, t.tw_grid, t.row_tiles, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n};
     const int nthr = gr_fft_threads(g.M);
     int cw = kRowsPerBlock;  // rows side by side: a power of two dividing M, within 1024 threads and the LDS
     while (cw > 1 && ((long)cw * nthr > 1024 || (long)cw * g.M * 8 > 163840 || g.M % cw)) cw /= 2;
