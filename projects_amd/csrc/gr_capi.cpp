@@ -137,6 +137,23 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
     const auto t0 = std::chrono::steady_clock::now();
     const GrGeom &g = pl->g;
     const int n_ang = pl->prm.n_ang;
+    size_t sop_bytes = 0;
+    auto free_sop = [&]() {
+        for (void *p : {(void *)pl->d_sop_tile, (void *)pl->d_sop_ch, (void *)pl->d_sop_keys, (void *)pl->d_sop_ent,
+                        (void *)pl->d_hmeta, (void *)pl->d_hseg, (void *)pl->d_hsplit, (void *)pl->d_hitems})
+            if (p) (void)hipFree(p);
+        pl->d_sop_tile = nullptr;
+        pl->d_sop_ch = nullptr;
+        pl->d_sop_keys = pl->d_sop_ent = nullptr;
+        pl->d_hmeta = nullptr;
+        pl->d_hseg = nullptr;
+        pl->d_hsplit = nullptr;
+        pl->d_hitems = nullptr;
+        pl->n_hitems = pl->n_split = pl->split_groups = 0;
+        pl->dev_bytes -= sop_bytes;
+        sop_bytes = 0;
+        pl->sop_chunks = 0;
+    };
     auto decline = [&](const char *why) {
         pl->gop_state = -1;
         (void)hipStreamSynchronize(s);
@@ -147,8 +164,19 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
                 pl->dev_bytes -= sizeof(unsigned) * pl->ntiles;
             }
         }
+        free_sop();
         pl->gop_aligned_tiles = pl->gop_heavy = 0;
         return required ? fail(GR_ENOMEM, std::string("gather operator: ") + why) : GR_OK;
+    };
+    auto dev_reserve = [&](void **p, size_t bytes) {
+        if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+        sop_bytes += bytes ? bytes : 1;
+        pl->dev_bytes += bytes ? bytes : 1;
+        return true;
     };
     pl->gop_state = -1;  // until it is built: a failed build is not tried again
     if (!gr_gop_fits(g, n_ang)) return decline("sample index range too large");
@@ -158,12 +186,22 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
     pl->dev_bytes += 2 * sizeof(unsigned) * nt;
     GrDevTables t = pl->tables();
     // count pass: rows of both layouts per tile (the aligned counts go to the
-    // offsets array, overwritten below)
-    GR_HIP(gr_launch_gop_count(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, s));
-    std::vector<unsigned> cnt(nt), cnt_al(nt), off(nt);
-    GR_HIP(hipMemcpyAsync(cnt.data(), pl->d_gop_n, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s));
-    GR_HIP(hipMemcpyAsync(cnt_al.data(), pl->d_gop_off, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s));
-    GR_HIP(hipStreamSynchronize(s));
+    // offsets array, overwritten below) and merge groups per tile
+    std::vector<unsigned> cnt(nt), cnt_al(nt), off(nt), groups(nt);
+    {
+        unsigned *d_groups = nullptr;
+        if (!dev_reserve((void **)&d_groups, sizeof(unsigned) * nt)) return decline("device allocation failed");
+        hipError_t e = gr_launch_gop_count(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, d_groups, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), pl->d_gop_n, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(cnt_al.data(), pl->d_gop_off, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(groups.data(), d_groups, sizeof(unsigned) * nt, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d_groups);
+        sop_bytes -= sizeof(unsigned) * nt;
+        pl->dev_bytes -= sizeof(unsigned) * nt;
+        if (e != hipSuccess) return hip_fail(e, "gather operator count pass");
+    }
     // per tile the aligned layout (wave-uniform merges) where it costs at
     // most a quarter more rows than the ragged one
     unsigned long long total = 0;
@@ -177,14 +215,71 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
         // the last tile of more than kGopHeavyRows rows take the
         // one-pair-per-wave path
         if ((cnt[i] & ~kGopAlignedBit) > kGopHeavyRows) pl->gop_heavy = std::max(pl->gop_heavy, i % pl->sector_len + 1);
+    }
+    // with compensated sums the heavy tiles are gridded in segments of
+    // kGopSegGroups merge groups (the merges are the cut points), all P pairs
+    // of a group per wave, their partials in the row transform's output
+    // buffer, which must hold them for every pair; otherwise one pair per wave
+    std::vector<uint4> hmeta;
+    std::vector<unsigned> hsplit;
+    unsigned long long seg_total = 0, part_total = 0;
+    if (pl->prm.accum == GR_ACCUM_COMPENSATED) {
+        hmeta.assign(nt, make_uint4(0u, 0u, 0u, 0u));
+        for (int i = 0; i < nt; i++) {
+            if (i % pl->sector_len >= pl->gop_heavy || groups[i] == 0) continue;
+            const unsigned S = (groups[i] + kGopSegGroups - 1) / kGopSegGroups;
+            hmeta[i] = make_uint4(S, (unsigned)seg_total, (unsigned)part_total, (unsigned)hsplit.size());
+            seg_total += 64ull * (S + 1);
+            part_total += groups[i];
+            hsplit.push_back((unsigned)i);
+        }
+        if (hsplit.empty() || part_total * 64 > (unsigned long long)g.M * g.out_n || seg_total > 0xffffffffull) {
+            hmeta.clear();
+            hsplit.clear();
+        }
+    }
+    // the heavy work items, slot by slot across the sectors (the centre-most
+    // tiles first): a cut tile's segments side by side, or per pair
+    std::vector<uint2> hitems;
+    for (int slot = 0; slot < pl->gop_heavy; slot++)
+        for (int sx = 0; sx < kGridSectors; sx++) {
+            const int i = sx * pl->sector_len + slot;
+            const unsigned S = hmeta.empty() ? (unsigned)kSpecGroup : hmeta[i].x;
+            for (unsigned k = 0; k < S; k++) hitems.push_back(make_uint2((unsigned)i, k));
+        }
+    // the staged form for the tiles past the heavy slots, where the plan
+    // admits it: count pass, exclusive sums of extra chunks and rows, fill
+    // pass (below, once the budget is checked); those tiles then take no
+    // entries of the streamed form
+    size_t sop_need = 0;
+    unsigned long long sop_rows = 0, sop_ch = nt;
+    std::vector<uint4> sop(nt);
+    if (gr_sop_fits(g, t, pl->prm.ltbl)) {
+        if (!dev_reserve((void **)&pl->d_sop_tile, sizeof(uint4) * nt)) return decline("device allocation failed");
+        GR_HIP(gr_launch_sop_count(g, t, n_ang, pl->prm.ltbl, pl->gop_heavy, pl->d_sop_tile, s));
+        GR_HIP(hipMemcpyAsync(sop.data(), pl->d_sop_tile, sizeof(uint4) * nt, hipMemcpyDeviceToHost, s));
+        GR_HIP(hipStreamSynchronize(s));
+        for (int i = 0; i < nt; i++) {
+            const uint4 c = sop[i];
+            sop[i] = make_uint4((unsigned)(sop_ch - nt), c.y, 0u, (unsigned)sop_rows);
+            sop_ch += c.y ? c.y - 1 : 0;
+            sop_rows += c.w;
+            if (i % pl->sector_len >= pl->gop_heavy) cnt[i] = 0;
+        }
+        sop_rows += 1;  // a spare row: the clamped entry loads of a chunk without rows stay inside
+        if (sop_ch * kSopKeysPerChunk > 0xffffffffull || sop_rows > 0xffffffffull / 64)
+            return decline("staged operator too large");
+        sop_need = (sizeof(uint2) + sizeof(unsigned) * kSopKeysPerChunk) * sop_ch + sizeof(unsigned) * 64ull * sop_rows;
+    }
+    for (int i = 0; i < nt; i++) {
         off[i] = (unsigned)total;
         total += 64ull * (cnt[i] & ~kGopAlignedBit);
         if (total > 0xffffffffull) return decline("more than 2^32 entries");
     }
-    GR_HIP(hipMemcpyAsync(pl->d_gop_n, cnt.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
     const size_t bytes = sizeof(uint2) * (size_t)std::max<unsigned long long>(total, 1);
     size_t free_b = 0, total_b = 0;
-    if (!required && (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes > free_b / 3 || bytes > max_bytes))
+    if (!required && (hipMemGetInfo(&free_b, &total_b) != hipSuccess || bytes + sop_need > free_b / 3 ||
+                      bytes + sop_need > max_bytes))
         return decline("larger than a third of the free device memory or the caller's budget");
     if (hipMalloc((void **)&pl->d_gop, bytes) != hipSuccess) {
         pl->d_gop = nullptr;
@@ -192,9 +287,53 @@ int build_gop(gr_plan *pl, hipStream_t s, bool required, size_t max_bytes) {
         return decline("device allocation failed");
     }
     pl->dev_bytes += bytes;
-    pl->gop_bytes = bytes + 2 * sizeof(unsigned) * nt;
+    if (pl->d_sop_tile) {
+        if (!dev_reserve((void **)&pl->d_sop_ch, sizeof(uint2) * sop_ch) ||
+            !dev_reserve((void **)&pl->d_sop_keys, sizeof(unsigned) * kSopKeysPerChunk * sop_ch) ||
+            !dev_reserve((void **)&pl->d_sop_ent, sizeof(unsigned) * 64ull * sop_rows)) {
+            (void)hipFree(pl->d_gop);
+            pl->d_gop = nullptr;
+            pl->dev_bytes -= bytes;
+            return decline("device allocation failed");
+        }
+        pl->sop_chunks = (int)std::min<unsigned long long>(sop_ch, INT32_MAX);
+        GR_HIP(hipMemcpyAsync(pl->d_sop_tile, sop.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice, s));
+        // the spare entry row: slot 0
+        static const unsigned zeros[64] = {};
+        GR_HIP(hipMemcpyAsync(pl->d_sop_ent + 64ull * (sop_rows - 1), zeros, sizeof zeros, hipMemcpyHostToDevice, s));
+        GR_HIP(gr_launch_sop_fill(g, t, n_ang, pl->prm.ltbl, pl->gop_heavy, pl->d_sop_tile, pl->d_sop_ch,
+                                  pl->d_sop_keys, pl->d_sop_ent, s));
+    }
+    if (!hitems.empty()) {
+        if (!dev_reserve((void **)&pl->d_hitems, sizeof(uint2) * hitems.size())) {
+            (void)hipFree(pl->d_gop);
+            pl->d_gop = nullptr;
+            pl->dev_bytes -= bytes;
+            return decline("device allocation failed");
+        }
+        GR_HIP(hipMemcpyAsync(pl->d_hitems, hitems.data(), sizeof(uint2) * hitems.size(), hipMemcpyHostToDevice, s));
+        pl->n_hitems = (int)hitems.size();
+    }
+    if (!hsplit.empty()) {
+        if (!dev_reserve((void **)&pl->d_hmeta, sizeof(uint4) * nt) ||
+            !dev_reserve((void **)&pl->d_hseg, sizeof(uint2) * seg_total) ||
+            !dev_reserve((void **)&pl->d_hsplit, sizeof(unsigned) * hsplit.size())) {
+            (void)hipFree(pl->d_gop);
+            pl->d_gop = nullptr;
+            pl->dev_bytes -= bytes;
+            return decline("device allocation failed");
+        }
+        GR_HIP(hipMemcpyAsync(pl->d_hmeta, hmeta.data(), sizeof(uint4) * nt, hipMemcpyHostToDevice, s));
+        GR_HIP(hipMemcpyAsync(pl->d_hsplit, hsplit.data(), sizeof(unsigned) * hsplit.size(), hipMemcpyHostToDevice,
+                              s));
+        pl->n_split = (int)hsplit.size();
+        pl->split_groups = (int)part_total;
+    }
+    pl->gop_bytes = bytes + 2 * sizeof(unsigned) * nt + sop_bytes;
+    GR_HIP(hipMemcpyAsync(pl->d_gop_n, cnt.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
     GR_HIP(hipMemcpyAsync(pl->d_gop_off, off.data(), sizeof(unsigned) * nt, hipMemcpyHostToDevice, s));
-    GR_HIP(gr_launch_gop_fill(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, pl->d_gop, s));
+    GR_HIP(gr_launch_gop_fill(g, t, n_ang, pl->prm.ltbl, pl->d_gop_n, pl->d_gop_off, pl->d_gop, pl->d_hmeta,
+                              pl->d_hseg, kGopSegGroups, s));
     GR_HIP(hipStreamSynchronize(s));
     pl->gop_state = 1;
     pl->gop_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -255,7 +394,9 @@ void free_plan(gr_plan *pl) {
     if (!pl) return;
     free_staging(pl);
     void *ptrs[] = {pl->d_cose, pl->d_sine, pl->d_roi, pl->d_wtbl, pl->d_winv, pl->d_filphase, pl->d_akey,
-                    pl->d_aord, pl->d_ang, pl->d_gop, pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl, pl->d_rowtiles};
+                    pl->d_aord, pl->d_ang, pl->d_gop, pl->d_gop_n, pl->d_gop_off, pl->d_dwtbl, pl->d_rowtiles,
+                    pl->d_sop_tile, pl->d_sop_ch, pl->d_sop_keys, pl->d_sop_ent, pl->d_hmeta, pl->d_hseg,
+                    pl->d_hsplit, pl->d_hitems};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     shared_release(pl->d_tw_row);
@@ -653,8 +794,8 @@ int recon_chunks(gr_plan *pl, const float *sino, long sino_slice_stride, long si
         }
         {
             KernelTimer kt(pl, s, 1);
-            GR_HIP(gr_launch_grid(g, t, pl->prm.n_ang, pl->prm.ltbl, pl->d_spec1, pl->d_spec2, pl->d_ht, np,
-                                  pl->prm.accum, s));
+            GR_HIP(gr_launch_grid(g, t, pl->prm.n_ang, pl->prm.ltbl, pl->d_spec1, pl->d_spec2, pl->d_ht, pl->d_z,
+                                  np, pl->prm.accum, s));
             kt.done();
         }
         {

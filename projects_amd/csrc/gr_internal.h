@@ -57,6 +57,27 @@ struct GrDevTables {
     const uint2 *gop = nullptr;
     const unsigned *gop_n = nullptr, *gop_off = nullptr;
     int gop_heavy = 0;  // leading slots of every sector gridded one pair per wave
+    // the heavy work: items {tile slot, segment}, the centre-most slots first;
+    // heavy tiles cut into segments of kGopSegGroups merge groups, gridded by
+    // one wave per (segment, pair) into partials that k_gop_merge sums:
+    // per tile {segments (0/1: not cut), first hseg entry, first partial
+    // group, index in hsplit}; per segment boundary and lane {row, merges};
+    // the cut tiles' slots
+    const uint2 *hitems = nullptr;
+    int n_hitems = 0;
+    const uint4 *hmeta = nullptr;
+    const uint2 *hseg = nullptr;
+    const unsigned *hsplit = nullptr;
+    int n_split = 0;       // cut tiles
+    int split_groups = 0;  // partial groups of all cut tiles
+    // the staged form of the operator for the other tiles (null: they use the
+    // entries above): per tile {first extra chunk, chunks, 0, first row}; per
+    // chunk {rows | merge bit, slots} and 128 keys (the records its LDS slots
+    // take), chunk 0 of tile slot t at index t; per chunk row 64 entries
+    // (slot | iu << 8 | iv << 20)
+    const uint4 *sop_tile = nullptr;
+    const uint2 *sop_ch = nullptr;
+    const unsigned *sop_keys = nullptr, *sop_ent = nullptr;
     // per tile row (M / 8 rows) the interval [lo, hi] of tiles that can hold
     // a nonzero cell (gr_active_tile_range), evaluated once on the device
     const int2 *row_tiles = nullptr;
@@ -104,8 +125,10 @@ constexpr int kGridSectors = 8;
 constexpr unsigned kGridNoTile = 0xffffffffu;
 // With t.gop set, K2 streams the precomputed operator instead of evaluating
 // the geometry; the images are bitwise the same.
+// `scratch` (pairs * M * out_n complex, free while K2 runs: the row
+// transform's output) holds the partials of heavy tiles gridded in segments.
 hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
-                          const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
+                          const float2 *spec1, const float2 *spec2, float2 *ht, float2 *scratch, int pairs,
                           int accum, hipStream_t s);
 
 // The precomputed operator: whether the geometry's sample indices fit its
@@ -116,9 +139,25 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
 constexpr unsigned kGopAlignedBit = 1u << 31;
 bool gr_gop_fits(const GrGeom &g, int n_ang);
 hipError_t gr_launch_gop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
-                               unsigned *tile_na, hipStream_t s);
+                               unsigned *tile_na, unsigned *tile_g, hipStream_t s);
+// hmeta/hseg (may be null): the tiles gridded in segments (below) get their
+// per-lane segment boundaries in hseg.
 hipError_t gr_launch_gop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
-                              const unsigned *tile_off, uint2 *gop, hipStream_t s);
+                              const unsigned *tile_off, uint2 *gop, const uint4 *hmeta, uint2 *hseg, int gps,
+                              hipStream_t s);
+// Merge groups per segment of a heavy tile gridded in segments.
+constexpr int kGopSegGroups = 1;
+// The staged operator of the tiles past the first n_heavy slots of every
+// sector: whether the plan admits it (nearest-neighbour convolvent, table
+// indices within its entries); the count pass ({0, chunks, 0, rows} per tile
+// into tile); the fill pass (tile then holds {first extra chunk, chunks, 0,
+// first row}; ch and keys have ntiles + the extra chunks' entries).
+bool gr_sop_fits(const GrGeom &g, const GrDevTables &t, long ltbl);
+hipError_t gr_launch_sop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, int n_heavy,
+                               uint4 *tile, hipStream_t s);
+hipError_t gr_launch_sop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, int n_heavy,
+                              const uint4 *tile, uint2 *ch, unsigned *keys, unsigned *ent, hipStream_t s);
+constexpr int kSopKeysPerChunk = 128;
 
 // K3: FFT (-1) of each row of Ht (over a), output rows of the window only:
 // Z[pairs][M][out_n].

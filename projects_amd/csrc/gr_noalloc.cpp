@@ -101,7 +101,7 @@ int gr_recon_device_noalloc(gr_plan *pl, const float *sino, long sino_slice_stri
         const int np = n_pairs - p0 < mp ? n_pairs - p0 : mp;
         NA_HIP(gr_launch_rowfft(g, t, pl->prm.n_ang, sino, sino_slice_stride, sino_row_stride, nullptr, n_slices, p0,
                                 np, spec1, spec2, s));
-        NA_HIP(gr_launch_grid(g, t, pl->prm.n_ang, pl->prm.ltbl, spec1, spec2, ht, np, pl->prm.accum, s));
+        NA_HIP(gr_launch_grid(g, t, pl->prm.n_ang, pl->prm.ltbl, spec1, spec2, ht, z, np, pl->prm.accum, s));
         NA_HIP(gr_launch_ifft_rows(g, t, ht, z, np, s));
         NA_HIP(gr_launch_ifft_cols(g, t, z, out, out_slice_stride, out_row_stride, n_slices, p0, np,
                                    pl->cols_per_block, s));

@@ -32,6 +32,17 @@ struct gr_plan {
     int gop_state = 0;
     int gop_aligned_tiles = 0;  // tiles in the aligned layout
     int gop_heavy = 0;          // leading slots of each sector gridded one pair per wave
+    // its staged form for the other tiles (null when the plan has none)
+    uint4 *d_sop_tile = nullptr;
+    uint2 *d_sop_ch = nullptr;
+    unsigned *d_sop_keys = nullptr, *d_sop_ent = nullptr;
+    int sop_chunks = 0;
+    // heavy work items and tiles gridded in segments (GrDevTables::hitems)
+    uint2 *d_hitems = nullptr;
+    uint4 *d_hmeta = nullptr;
+    uint2 *d_hseg = nullptr;
+    unsigned *d_hsplit = nullptr;
+    int n_hitems = 0, n_split = 0, split_groups = 0;
     long pairs_done = 0;  // pairs reconstructed so far (the automatic build's trigger)
     // workspace for max_pairs pairs: owned, or a region bound by a queue
     int max_pairs = 1;
@@ -82,6 +93,17 @@ struct gr_plan {
             t.gop_n = d_gop_n;
             t.gop_off = d_gop_off;
             t.gop_heavy = gop_heavy;
+            t.hitems = d_hitems;
+            t.n_hitems = n_hitems;
+            t.hmeta = d_hmeta;
+            t.hseg = d_hseg;
+            t.hsplit = d_hsplit;
+            t.n_split = n_split;
+            t.split_groups = split_groups;
+            t.sop_tile = d_sop_tile;
+            t.sop_ch = d_sop_ch;
+            t.sop_keys = d_sop_keys;
+            t.sop_ent = d_sop_ent;
         }
         return t;
     }

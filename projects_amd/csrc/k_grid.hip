@@ -50,6 +50,8 @@
 #include "gr_gather.hpp"
 #include "gr_internal.h"
 
+#include <algorithm>
+
 #pragma clang fp contract(off)
 
 namespace {
@@ -326,15 +328,19 @@ __device__ __forceinline__ unsigned wave_max(unsigned v) {
 // tile_n's top bit names (its rows in the low bits).
 template <int MODE>
 __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n, unsigned *tile_na,
-                                                   const unsigned *tile_off, uint2 *gop) {
+                                                   const unsigned *tile_off, uint2 *gop, unsigned *tile_g,
+                                                   const uint4 *hmeta, uint2 *hseg, int gps) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
     if (ti >= A.ntiles) return;
     const unsigned tt = A.tiles[ti];
     if (tt == kGridNoTile) {  // padding slot: no rows
-        if (MODE == 0 && lane == 0) tile_n[ti] = tile_na[ti] = 0;
+        if (MODE == 0 && lane == 0) tile_n[ti] = tile_na[ti] = tile_g[ti] = 0;
         return;
     }
+    // tiles left without rows (no taps, or listed in the staged operator) are
+    // not filled
+    if (MODE == 1 && (tile_n[ti] & kGopRows) == 0) return;
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
     const int M = A.g.M;
@@ -348,9 +354,18 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
     unsigned g0 = 0;  // aligned: first row of the open group
     unsigned ng = 0;  // taps of this lane in the open group
     unsigned na = 0;  // count pass: rows of the aligned layout
+    unsigned gc = 0;  // merge points passed
+    unsigned nm = 0;  // merge marks in this lane's entries so far
     uint2 pend = make_uint2(0u, 0u);
     bool has_pend = false;  // the lane's latest tap, not yet written (it may get the merge mark)
     auto put = [&](unsigned row, uint2 e) { gop[base + 64ul * row] = e; };
+    // a tile cut into segments of gps merge groups (hmeta.x >= 1): per
+    // segment boundary the lane's row and merge count
+    const uint4 hm = MODE == 1 && hseg ? hmeta[ti] : make_uint4(0u, 0u, 0u, 0u);
+    auto mark = [&](unsigned seg) {
+        if (hm.x >= 1 && seg <= hm.x) hseg[hm.y + 64ul * seg + lane] = make_uint2(aligned ? g0 : n, nm);
+    };
+    mark(0);
     auto close_group = [&]() {  // a merge point of k_grid (wave-uniform)
         if (MODE == 0) {
             na += wave_max(ng);
@@ -359,11 +374,14 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
             if (has_pend) put(g0 + ng - 1, ng == len ? make_uint2(pend.x | kGopMerge, pend.y) : pend);
             for (unsigned r = ng; r < len; r++) put(g0 + r, make_uint2(r + 1 == len ? kGopMerge : 0u, 0u));
             g0 += len;
+            nm += len ? 1u : 0u;
         } else if (has_pend) {
             put(n - 1, make_uint2(pend.x | kGopMerge, pend.y));
+            nm++;
         }
         has_pend = false;
         ng = 0;
+        if (++gc % (unsigned)max(gps, 1) == 0) mark(gc / (unsigned)max(gps, 1));
     };
     int lines = 0;
     walk_tile(
@@ -394,17 +412,135 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
         if (lane == 0) {
             tile_n[ti] = n;
             tile_na[ti] = na;
+            tile_g[ti] = gc;
         }
     } else {
+        mark(hm.x);  // the end of the last segment
         const unsigned rows = tile_n[ti] & kGopRows;
         for (unsigned r = aligned ? g0 : n; r < rows; r++) put(r, make_uint2(0u, 0u));
     }
+}
+
+// ---- the staged operator --------------------------------------------------------
+//
+// Where the operator above hands each lane a sample record per tap -- a
+// 64-byte gather per lane and row, 64 records per wave instruction, which
+// the texture addresser takes one cache line at a time -- a tile's records
+// are reused by ~8 of its cells each.  The staged form loads every record a
+// tile needs once into the wave's LDS and lets the taps read it there.
+//
+// A tile's taps are cut into chunks at k_grid's merge points (every
+// kMergeAngles sample lines, and the last line) and wherever the next run of
+// samples would overflow the chunk's kSopSlots LDS slots.  Per chunk: a
+// header {rows | merge bit, slots}, its slots' records (kSopSlots keys
+// idx | mirror << 30 as the operator's entries, the unused ones zero; slot
+// 0 is record 0 of spec1, which K1 writes as zero), and its rows: per lane
+// its taps within the chunk in k_grid's order, the lanes with fewer taps
+// (and the rows up to a multiple of kSopRowBatch) padded with slot 0 (zero
+// times a weight adds nothing to a partial that never holds -0), each entry slot | iu << 8 | iv << 20 with the weight
+// wtbl[iu] * wtbl[iv] read from LDS -- k_grid's product.  A chunk that ends
+// at a merge point carries the merge bit: the merge is wave-uniform.  So per
+// cell the float operations and their order are k_grid's: bitwise its images.
+//
+// Chunk 0 of tile slot ti is chunk ti (header ch[ti], keys at 128 ti), so a
+// wave can fetch it with the tile's offsets before it knows them; a tile's
+// further chunks are ntiles + its first extra chunk + k - 1.
+constexpr int kSopSlots = kSopKeysPerChunk;
+constexpr unsigned kSopMerge = 1u << 31;
+constexpr int kSopRowBatch = 4;  // a chunk's rows: a multiple of this (padded), added a batch at a time
+
+// MODE 0 (count): {0, chunks, 0, rows} of each tile past the first n_heavy
+// slots of its sector (zeros elsewhere).  MODE 1 (fill): headers, keys and
+// entries; tile holds {first extra chunk, chunks, 0, first row}.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_sop_build(GridArgs A, int n_heavy, uint4 *tile, uint2 *ch, unsigned *keys,
+                                                   unsigned *ent) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (ti >= A.ntiles) return;
+    const unsigned tt = A.tiles[ti];
+    if (tt == kGridNoTile || ti % A.sector_len < n_heavy) {
+        if (MODE == 0 && lane == 0) tile[ti] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
+    const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+    const int M = A.g.M;
+    const float M2 = A.g.fM2;
+    const bool valid = a >= 1 && b >= 1 && a < M && b < M;
+    const float pa = (float)a, pb = (float)b, ma = (float)(M - a), mb = (float)(M - b);
+    const uint4 to = MODE == 1 ? tile[ti] : make_uint4(0u, 0u, 0u, 0u);
+    unsigned nch = 0, nrows = 0;  // closed chunks and their rows (wave-uniform)
+    unsigned slots = 1;           // the open chunk's slots (slot 0: the zero record)
+    unsigned ntc = 0;             // this lane's taps in the open chunk
+    auto chunk_index = [&]() { return nch ? (unsigned)A.ntiles + to.x + nch - 1u : (unsigned)ti; };
+    auto close = [&](bool merge_after) {  // wave-uniform
+        const unsigned rows = (wave_max(ntc) + kSopRowBatch - 1) & ~(unsigned)(kSopRowBatch - 1);
+        if (MODE == 1) {
+            const unsigned long e0 = 64ul * (to.w + nrows) + lane;
+            for (unsigned r = ntc; r < rows; r++) ent[e0 + 64ul * r] = 0u;
+            const unsigned ci = chunk_index();
+            for (unsigned k = slots + lane; k < (unsigned)kSopSlots; k += 64) keys[(unsigned long)ci * kSopSlots + k] = 0u;
+            if (lane == 0) {
+                ch[ci] = make_uint2(rows | (merge_after ? kSopMerge : 0u), slots);
+                keys[(unsigned long)ci * kSopSlots] = 0u;
+            }
+        }
+        nch++;
+        nrows += rows;
+        slots = 1;
+        ntc = 0;
+    };
+    int lines = 0;
+    walk_tile(
+        A, tx, ty, valid, pa - M2, pb - M2,
+        [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
+            if (slots + (unsigned)nj > (unsigned)kSopSlots) close(false);
+            const unsigned sb = slots - (unsigned)c0;  // slot of sample j: sb + j
+            if (MODE == 1 && lane < nj)
+                keys[(unsigned long)chunk_index() * kSopSlots + slots + lane] =
+                    (unsigned)(row_off + c0 + lane) | (mirror ? kGopMirror : 0u);
+            slots += (unsigned)nj;
+            const float qa = mirror ? ma : pa, qb = mirror ? mb : pb;
+            for (int j = j0; j <= j1; j++) {
+                int iu, iv;
+                if (gr_tap_index(j, C, S, qa, qb, A.g, iu, iv)) {
+                    if (MODE == 1)
+                        ent[64ul * (to.w + nrows + ntc) + lane] =
+                            (sb + (unsigned)j) | (unsigned)iu << 8 | (unsigned)iv << 20;
+                    ntc++;
+                }
+            }
+        },
+        [&]() {
+            if (++lines == kMergeAngles) {
+                lines = 0;
+                close(true);
+            }
+        });
+    if (lines) close(true);  // k_grid's final merge
+    if (MODE == 0 && lane == 0) tile[ti] = make_uint4(0u, nch, 0u, nrows);
 }
 
 struct GopArgs {
     const uint2 *gop;
     const unsigned *tile_n, *tile_off;
     int n_heavy;  // slots [0, n_heavy) of every sector take the one-pair-per-wave path
+    // the staged operator of the other slots (null: they stream the entries)
+    const uint4 *sop_tile;
+    const uint2 *sop_ch;
+    const unsigned *sop_keys, *sop_ent;
+    int sop_waves;  // persistent light waves per sector
+    // the heavy work items (tile, segment), heavy tiles cut into segments
+    // (GrDevTables), and their partials
+    const uint2 *hitems;
+    int n_hitems;
+    const uint4 *hmeta;
+    const uint2 *hseg;
+    const unsigned *hsplit;
+    int n_split;
+    float2 *part;
+    int part_pairs;
 };
 
 __device__ __forceinline__ uint2 gop_load(const uint2 *p) {
@@ -487,42 +623,180 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
     store_cells<ACCUM, 1>(A, pair, 1, a, b, acc);
 }
 
-// Workgroups of k_grid_op given to the one-pair-per-wave path (a multiple of
-// the sector count, so the other workgroups keep their XCD-to-sector map).
+// A wave's record slots in LDS: plane q holds piece q of every slot (16
+// bytes of the P pairs' values for P >= 2, the real and the imaginary float
+// for P = 1), so the 16-byte reads of 64 lanes at random slots spread over
+// all banks (a 64-byte record stride would leave them a quarter of them).
+// Records reach the slots by global-to-LDS loads, which write lane l's piece
+// at the load's LDS base + l * piece: slot l (or l + 64) of each plane, no
+// registers in between.  (Loading whole records per lane group instead --
+// coalesced reads, pieces rotated within a slot against bank conflicts --
+// took fewer texture-addresser cycles but more VALU work and was slower.)
 template <int P>
-__host__ __device__ __forceinline__ int heavy_blocks(int n_heavy) {
-    return (kGridSectors * n_heavy * P + 4 * kGridSectors - 1) / (4 * kGridSectors) * kGridSectors;
+struct SopStage {
+    static constexpr int piece = P >= 2 ? 16 : 4;  // bytes per lane and load
+    static constexpr int planes = P * 8 / piece;
+    static constexpr int plane_bytes = kSopSlots * piece;
+};
+
+template <int P>
+__device__ __forceinline__ void sop_stage(unsigned key, const float2 *base1, const float2 *base2, char *stg,
+                                          int slot0) {
+    const char *rec = reinterpret_cast<const char *>(((key & kGopMirror) ? base2 : base1) +
+                                                     (long)(key & kGopIdx) * kSpecGroup);
+#if defined(__HIP_DEVICE_COMPILE__)  // a device builtin: the host pass sees no body
+    using St = SopStage<P>;
+#pragma unroll
+    for (int q = 0; q < St::planes; q++)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)(rec + q * St::piece),
+            (__attribute__((address_space(3))) void *)(stg + q * St::plane_bytes + slot0 * St::piece), St::piece,
+            0, 0);
+#else
+    (void)rec;
+    (void)stg;
+    (void)slot0;
+#endif
 }
 
-// k_grid's accumulation, fed by the operator's entries.  The first
-// heavy_blocks workgroups take slots [0, n_heavy) of every sector one pair per
-// wave; the others take one tile per wave for the P pairs of the group,
-// software-pipelined: entries are loaded four rows ahead, the samples of row
-// k+1 are gathered while row k is added.  Rows past a tile's end are zero
-// entries.
-template <int ACCUM, int P>
-__global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int pair0 = A.pair_base + blockIdx.y * P;
-    const int nvalid = min(P, A.pair_end - pair0);
-    const int hb = heavy_blocks<P>(O.n_heavy);
-    if ((int)blockIdx.x < hb) {
-        // wave hw: sector hw / (n_heavy P), slot (hw / P) mod n_heavy, pair hw mod P
-        const int hw = blockIdx.x * 4 + wave;
-        const int sx = hw / (O.n_heavy * P), r = hw % (O.n_heavy * P);
-        const int slot = r / P, p = r % P;
-        if (sx >= kGridSectors || p >= nvalid) return;
-        const int ti = sx * A.sector_len + slot;
-        if (A.tiles[ti] != kGridNoTile) grid_op_heavy<ACCUM>(A, O, ti, pair0 + p, lane);
-        return;
+template <int P>
+__device__ __forceinline__ void sop_read(const char *stg, unsigned slot, float2 (&v)[P]) {
+    using St = SopStage<P>;
+    if (P >= 2) {
+#pragma unroll
+        for (int q = 0; q < P / 2; q++) {
+            const float4 x = *reinterpret_cast<const float4 *>(stg + q * St::plane_bytes + 16 * slot);
+            v[2 * q] = make_float2(x.x, x.y);
+            v[2 * q + 1] = make_float2(x.z, x.w);
+        }
+    } else {
+        v[0] = make_float2(*reinterpret_cast<const float *>(stg + 4 * slot),
+                           *reinterpret_cast<const float *>(stg + St::plane_bytes + 4 * slot));
     }
-    const int ti = sector_slot(A, blockIdx.x - hb, 4, wave, O.n_heavy);
-    if (ti < 0) return;
+}
+
+// The light tiles of one sector for the P pairs of a group from the staged
+// operator, persistent: the wave takes slots slot0, slot0 + stride, ... and
+// loads the next slot's tile, offsets, first chunk header and keys while it
+// works on the current one (they sit at positions fixed by the slot), so a
+// tile costs one round trip -- its records into the slots and its first
+// entries -- plus its arithmetic.  Within a tile the next chunk's header and
+// keys load while a chunk's rows are added.  The entries array ends in a
+// spare row, so the clamped entry loads of a chunk without rows stay inside.
+template <int ACCUM, int P>
+__device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs &O, int sector, int slot0,
+                                               int stride, int pair0, int nvalid, int lane, char *stg,
+                                               const float *s_w) {
+    const long group = spec_group_offset(A, pair0);
+    const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
+    const int first = sector * A.sector_len;
+    struct Next {
+        unsigned tt;
+        uint4 td;
+        uint2 h;
+        unsigned k0, k1;
+    };
+    auto fetch = [&](int slot) {  // slot < sector_len
+        const int ti = first + slot;
+        Next n;
+        n.tt = A.tiles[ti];
+        n.td = O.sop_tile[ti];
+        n.h = O.sop_ch[ti];
+        n.k0 = O.sop_keys[(unsigned long)ti * kSopSlots + lane];
+        n.k1 = O.sop_keys[(unsigned long)ti * kSopSlots + 64 + lane];
+        return n;
+    };
+    if (slot0 >= A.sector_len) return;
+    Next nx = fetch(slot0);
+    for (int slot = slot0; slot < A.sector_len; slot += stride) {
+        const Next cur = nx;
+        nx = fetch(min(slot + stride, A.sector_len - 1));
+        if (cur.tt == kGridNoTile) continue;
+        const int tx = (int)(cur.tt & 0xffffu), ty = (int)(cur.tt >> 16);
+        const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+        float4 acc[P];
+        float2 part[P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            acc[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            part[p] = make_float2(0.0f, 0.0f);
+        }
+        auto consume = [&](unsigned e) {
+            const float w = s_w[(e >> 8) & 0xfffu] * s_w[e >> 20];
+            float2 v[P];
+            sop_read<P>(stg, e & 0xffu, v);
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                if (ACCUM == GR_ACCUM_COMPENSATED) {
+                    part[p].x = __builtin_fmaf(w, v[p].x, part[p].x);
+                    part[p].y = __builtin_fmaf(w, v[p].y, part[p].y);
+                } else {
+                    acc[p].x += w * v[p].x;
+                    acc[p].z += w * v[p].y;
+                }
+            }
+        };
+        const unsigned nch = cur.td.y, extra = (unsigned)A.ntiles + cur.td.x;
+        const unsigned *ep = O.sop_ent + 64ul * cur.td.w + lane;
+        uint2 h = cur.h;
+        unsigned k0 = cur.k0, k1 = cur.k1;
+        for (unsigned c = 0; c < nch; c++) {
+            const unsigned rows = h.x & ~kSopMerge, ns = h.y;
+            const unsigned last = rows ? rows - 1u : 0u;
+            auto eload = [&](unsigned k) { return __builtin_nontemporal_load(ep + 64ul * min(k, last)); };
+            sop_stage<P>(k0, base1, base2, stg, 0);
+            if (ns > 64u) sop_stage<P>(k1, base1, base2, stg, 64);
+            // the next chunk (or this one again past the last)
+            const unsigned cn = c + 1 < nch ? extra + c : (c ? extra + c - 1 : (unsigned)(first + slot));
+            const uint2 hn = O.sop_ch[cn];
+            const unsigned kn0 = O.sop_keys[(unsigned long)cn * kSopSlots + lane];
+            const unsigned kn1 = O.sop_keys[(unsigned long)cn * kSopSlots + 64 + lane];
+            // rows come in whole batches of kSopRowBatch: no per-row guard,
+            // so a batch's LDS reads overlap
+            constexpr int RB = kSopRowBatch;
+            unsigned t[RB], u[RB];
+#pragma unroll
+            for (int i = 0; i < RB; i++) t[i] = eload(i);
+            __builtin_amdgcn_wave_barrier();
+            for (unsigned r0 = 0; r0 < rows; r0 += 2 * RB) {
+#pragma unroll
+                for (int i = 0; i < RB; i++) u[i] = eload(r0 + RB + i);
+#pragma unroll
+                for (int i = 0; i < RB; i++) consume(t[i]);
+                if (r0 + RB >= rows) break;
+#pragma unroll
+                for (int i = 0; i < RB; i++) t[i] = eload(r0 + 2 * RB + i);
+#pragma unroll
+                for (int i = 0; i < RB; i++) consume(u[i]);
+            }
+            ep += 64ul * rows;
+            if (ACCUM == GR_ACCUM_COMPENSATED && (h.x & kSopMerge)) merge<P>(acc, part);
+            h = hn;
+            k0 = kn0;
+            k1 = kn1;
+            __builtin_amdgcn_wave_barrier();  // this chunk's reads before the next chunk's records
+        }
+        store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
+    }
+}
+
+// One tile (seg < 0) or one segment of a cut heavy tile for the P pairs of a
+// group from the streamed entries, software-pipelined: entries are loaded
+// four rows ahead, the samples of row k+1 are gathered while row k is added.
+// A whole tile runs its rows to the end, merging into its sums, and stores
+// its cells.  A segment runs each lane's rows between its marks at the
+// segment's ends (segments start at merge points, so they hold whole merge
+// groups), rows past the lane's end reading as zero entries (sample 0,
+// weight 0, no merge; loads stay inside the tile's rows), and each merge
+// stores the finished partials to partial slot (first partial group + the
+// lane's merge count) instead of adding them to sums: k_gop_merge sums
+// them in order -- the same float operations, cut at merges.
+template <int ACCUM, int P>
+__device__ __forceinline__ void grid_op_tile(const GridArgs &A, const GopArgs &O, int ti, int seg, int pair0,
+                                             int nvalid, int lane) {
     const unsigned tt = A.tiles[ti];
-    if (tt == kGridNoTile) return;
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
     const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
-
     float4 acc[P];
     float2 part[P];
 #pragma unroll
@@ -533,10 +807,45 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
     const long group = spec_group_offset(A, pair0);
     const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
     const uint2 *e = O.gop + (unsigned long)O.tile_off[ti] + lane;
-    const unsigned n = O.tile_n[ti] & kGopRows;
+    const unsigned rows = O.tile_n[ti] & kGopRows;
+    unsigned r0 = 0, len = rows, m = 0;
+    float2 *out = nullptr;
+    const size_t gstride = (size_t)O.part_pairs * 64;
+    if (seg >= 0) {
+        const uint4 hm = O.hmeta[ti];
+        const uint2 m0 = O.hseg[hm.y + 64ul * seg + lane], m1 = O.hseg[hm.y + 64ul * (seg + 1) + lane];
+        r0 = m0.x;
+        len = m1.x - m0.x;
+        m = hm.z + m0.y;
+        out = O.part + (size_t)pair0 * 64 + lane;
+    }
+    const unsigned n = seg >= 0 ? wave_max(len) : rows;
     if (n) {
         const uint2 zero = make_uint2(0u, 0u);
-        auto row = [&](unsigned k) { return gop_load(e + 64ul * min(k, n - 1)); };
+        auto row = [&](unsigned k) {
+            const uint2 t = gop_load(e + 64ul * min(r0 + k, rows - 1u));
+            return k < len ? t : zero;
+        };
+        auto consume = [&](uint2 t, const float2 (&v)[P]) {
+            if (seg < 0) {
+                gop_consume<ACCUM, P>(t, v, acc, part);
+                return;
+            }
+            const float w = __uint_as_float(t.y);
+#pragma unroll
+            for (int p = 0; p < P; p++) {
+                part[p].x = __builtin_fmaf(w, v[p].x, part[p].x);
+                part[p].y = __builtin_fmaf(w, v[p].y, part[p].y);
+            }
+            if (t.x & kGopMerge) {
+#pragma unroll
+                for (int p = 0; p < P; p++) {
+                    if (p < nvalid) out[m * gstride + 64 * p] = part[p];
+                    part[p] = make_float2(0.0f, 0.0f);
+                }
+                m++;
+            }
+        };
         uint2 t0 = row(0), t1 = row(1), t2 = row(2), t3 = row(3);
         float2 va[P], vb[P];
         gop_gather<P>(t0, base1, base2, va);
@@ -547,22 +856,131 @@ __global__ void __launch_bounds__(256) k_grid_op(GridArgs A, GopArgs O) {
             const uint2 c0 = t0;
             t0 = t2;
             t2 = row(k + 4);
-            gop_consume<ACCUM, P>(c0, va, acc, part);
+            consume(c0, va);
             gop_gather<P>(k + 2 < n ? t0 : zero, base1, base2, va);
             t1 = t3;
             t3 = row(k + 5);
-            gop_consume<ACCUM, P>(c1, vb, acc, part);
+            consume(c1, vb);
         }
     }
-    store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
+    if (seg < 0) store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
 }
+
+// The cut heavy tiles' sums: per (tile, pair) wave, each lane merges its
+// partials in order, as grid_op_heavy's merges would have (the merges of
+// groups where the lane had no tap add zero partials, which change no bit),
+// and stores its cell.  Partials load sixteen at a time.
+__global__ void __launch_bounds__(256) k_gop_merge(GridArgs A, GopArgs O, int pairs) {
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= O.n_split * pairs) return;
+    const int h = w / pairs, pair = w % pairs;
+    const unsigned ti = O.hsplit[h];
+    const unsigned tt = A.tiles[ti];
+    const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
+    const int a = tx * kTile + (lane & 7), b = ty * kTile + (lane >> 3);
+    const uint4 hm = O.hmeta[ti];
+    const unsigned total = O.hseg[hm.y + 64ul * hm.x + lane].y, n = wave_max(total);
+    const float2 *in = O.part + (size_t)hm.z * O.part_pairs * 64 + (size_t)pair * 64 + lane;
+    const size_t gstride = (size_t)O.part_pairs * 64;
+    float4 acc[1] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    for (unsigned k0 = 0; k0 < n; k0 += 16) {
+        float2 q[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) q[i] = in[min(k0 + i, total ? total - 1u : 0u) * gstride];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            float2 part[1] = {k0 + i < total ? q[i] : make_float2(0.0f, 0.0f)};
+            merge<1>(acc, part);
+        }
+    }
+    store_cells<GR_ACCUM_COMPENSATED, 1>(A, pair, 1, a, b, acc);
+}
+// Workgroups of k_grid_op given to the one-pair-per-wave path (a multiple of
+// the sector count, so the other workgroups keep their XCD-to-sector map).
+__host__ __device__ __forceinline__ int heavy_blocks(int n_items) {
+    return (n_items + 4 * kGridSectors - 1) / (4 * kGridSectors) * kGridSectors;
+}
+
+// k_grid's accumulation, fed by the operator.  The first heavy_blocks
+// workgroups take the heavy work items (slots [0, n_heavy) of every sector)
+// from the streamed entries: with compensated sums one segment of a cut tile
+// per wave for the P pairs of the group (grid_op_tile; k_gop_merge sums the
+// partials afterwards), otherwise a tile one pair per wave (grid_op_heavy).
+// The others take one tile per wave for the P pairs -- SOP: from the staged
+// operator (grid_sop_light), otherwise from the streamed entries
+// (grid_op_tile).
+template <int ACCUM, int P, bool SOP>
+__global__ void __launch_bounds__(256, 4) k_grid_op(GridArgs A, GopArgs O) {
+    extern __shared__ float smem[];  // staged operator: wtbl, then the waves' record slots
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (SOP) {
+        for (int i = threadIdx.x; i <= A.g.ltbl; i += blockDim.x) smem[i] = A.wtbl[i];
+        __syncthreads();
+    }
+    const int pair0 = A.pair_base + blockIdx.y * P;
+    const int nvalid = min(P, A.pair_end - pair0);
+    // SOP: the persistent light workgroups first (they start at once and
+    // hold most of the machine), the heavy items after them in the slots
+    // left; otherwise the heavy items first (their long chains start early)
+    const int hb = heavy_blocks(O.n_hitems);
+    const int nlight = SOP ? 2 * O.sop_waves : 0;  // kGridSectors * sop_waves / 4
+    const int hbx = (int)blockIdx.x - nlight;
+    if (hbx >= 0 && hbx < hb) {
+        // wave: work item hbx * 4 + wave -- a cut tile's segment (the P pairs
+        // into partials), or a heavy tile one pair per wave
+        const int item = hbx * 4 + wave;
+        const uint2 it = O.hitems[min(item, O.n_hitems - 1)];
+        const int ti = (int)it.x;
+        if (item >= O.n_hitems || A.tiles[ti] == kGridNoTile) return;
+        if (O.hmeta) {
+            grid_op_tile<ACCUM, P>(A, O, ti, (int)it.y, pair0, nvalid, lane);
+        } else {
+            const int p = (int)it.y;  // tiles not cut: one item per pair
+            if (p < nvalid) grid_op_heavy<ACCUM>(A, O, ti, pair0 + p, lane);
+        }
+        return;
+    }
+    if (SOP) {
+        // persistent: workgroup b (XCD b mod 8) serves sector b mod 8 with
+        // sop_waves waves, each taking every sop_waves-th light slot
+        const int wv = __builtin_amdgcn_readfirstlane(wave);
+        const int lb = blockIdx.x;
+        char *stg = reinterpret_cast<char *>(smem + A.lds_wtbl) + (size_t)wv * kSopSlots * P * sizeof(float2);
+        grid_sop_light<ACCUM, P>(A, O, lb % kGridSectors, O.n_heavy + (lb / kGridSectors) * 4 + wv, O.sop_waves,
+                                 pair0, nvalid, lane, stg, smem);
+        return;
+    }
+    const int ti = sector_slot(A, blockIdx.x - hb, 4, wave, O.n_heavy);
+    if (ti < 0) return;
+    const unsigned tt = A.tiles[ti];
+    if (tt == kGridNoTile) return;
+    grid_op_tile<ACCUM, P>(A, O, ti, -1, pair0, nvalid, lane);
+}
+
+// Persistent workgroups per sector (and group of pairs) of the staged light
+// path: 8 x 96 take three of the four workgroups an MI355X CU holds, the
+// fourth serves the heavy items.
+constexpr int kSopBlocksPerSector = 96;
 
 template <int ACCUM, int P>
 hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_base, int wpb, hipStream_t s) {
     if (O) {
         const int light = (A.sector_len - O->n_heavy + 3) / 4;
-        const int blocks = heavy_blocks<P>(O->n_heavy) + kGridSectors * light;
-        hipLaunchKernelGGL((k_grid_op<ACCUM, P>), dim3(blocks, groups), dim3(256), 0, s, A, *O);
+        const int blocks = heavy_blocks(O->n_hitems) + kGridSectors * light;
+        if (!O->sop_tile) {
+            hipLaunchKernelGGL((k_grid_op<ACCUM, P, false>), dim3(blocks, groups), dim3(256), 0, s, A, *O);
+            return hipGetLastError();
+        }
+        // the staged light tiles: persistent workgroups, kSopBlocksPerSector
+        // per sector and group, fewer when the sector has fewer slots
+        const int lb = std::min(light, kSopBlocksPerSector);
+        const int sblocks = heavy_blocks(O->n_hitems) + kGridSectors * lb;
+        GopArgs o = *O;
+        o.sop_waves = 4 * lb;
+        const size_t lds = lds_base + sizeof(float2) * 4 * kSopSlots * P;
+        hipError_t e = gr_allow_lds(k_grid_op<ACCUM, P, true>, lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_grid_op<ACCUM, P, true>), dim3(sblocks, groups), dim3(256), lds, s, A, o);
         return hipGetLastError();
     }
     const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
@@ -638,34 +1056,81 @@ GridArgs make_args(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl) 
 int gr_grid_tile_size() { return kTile; }
 
 hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl,
-                          const float2 *spec1, const float2 *spec2, float2 *ht, int pairs,
+                          const float2 *spec1, const float2 *spec2, float2 *ht, float2 *scratch, int pairs,
                           int accum, hipStream_t s) {
     GridArgs A = make_args(g, t, n_ang, ltbl);
     A.spec1 = spec1;
     A.spec2 = spec2;
     A.ht = ht;
-    GopArgs O{t.gop, t.gop_n, t.gop_off, t.gop_heavy};
+    // segments only with compensated sums (their merges are the cut points)
+    const bool cut = t.hmeta && t.n_split > 0 && accum == GR_ACCUM_COMPENSATED;
+    GopArgs O{};
+    O.gop = t.gop;
+    O.tile_n = t.gop_n;
+    O.tile_off = t.gop_off;
+    O.n_heavy = t.gop_heavy;
+    O.sop_tile = t.sop_tile;
+    O.sop_ch = t.sop_ch;
+    O.sop_keys = t.sop_keys;
+    O.sop_ent = t.sop_ent;
+    O.hitems = t.hitems;
+    O.n_hitems = t.n_hitems;
+    O.hmeta = cut ? t.hmeta : nullptr;
+    O.hseg = t.hseg;
+    O.hsplit = t.hsplit;
+    O.n_split = cut ? t.n_split : 0;
+    O.part = scratch;
+    O.part_pairs = pairs;
     const GopArgs *op = t.gop ? &O : nullptr;
     const size_t lds_base = sizeof(float) * A.lds_wtbl;
     const int wpb = g.grid_wpb == 1 || g.grid_wpb == 2 ? g.grid_wpb : 4;
-    if (accum == GR_ACCUM_PLAIN) return launch_all<GR_ACCUM_PLAIN>(A, op, pairs, lds_base, wpb, s);
-    return launch_all<GR_ACCUM_COMPENSATED>(A, op, pairs, lds_base, wpb, s);
+    hipError_t e = accum == GR_ACCUM_PLAIN ? launch_all<GR_ACCUM_PLAIN>(A, op, pairs, lds_base, wpb, s)
+                                           : launch_all<GR_ACCUM_COMPENSATED>(A, op, pairs, lds_base, wpb, s);
+    if (e != hipSuccess || !op || !cut) return e;
+    A.pair_base = 0;
+    A.pair_end = pairs;
+    const int waves = O.n_split * pairs;
+    hipLaunchKernelGGL(k_gop_merge, dim3((waves + 3) / 4), dim3(256), 0, s, A, O, pairs);
+    return hipGetLastError();
 }
 
 bool gr_gop_fits(const GrGeom &g, int n_ang) { return (long)n_ang * g.half <= (long)kGopIdx + 1; }
 
 hipError_t gr_launch_gop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
-                               unsigned *tile_na, hipStream_t s) {
+                               unsigned *tile_na, unsigned *tile_g, hipStream_t s) {
     const GridArgs A = make_args(g, t, n_ang, ltbl);
     hipLaunchKernelGGL(k_gop_build<0>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, tile_n, tile_na,
-                       (const unsigned *)nullptr, (uint2 *)nullptr);
+                       (const unsigned *)nullptr, (uint2 *)nullptr, tile_g, (const uint4 *)nullptr,
+                       (uint2 *)nullptr, 0);
     return hipGetLastError();
 }
 
 hipError_t gr_launch_gop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, unsigned *tile_n,
-                              const unsigned *tile_off, uint2 *gop, hipStream_t s) {
+                              const unsigned *tile_off, uint2 *gop, const uint4 *hmeta, uint2 *hseg, int gps,
+                              hipStream_t s) {
     const GridArgs A = make_args(g, t, n_ang, ltbl);
     hipLaunchKernelGGL(k_gop_build<1>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, tile_n, (unsigned *)nullptr,
-                       tile_off, gop);
+                       tile_off, gop, (unsigned *)nullptr, hmeta, hseg, gps);
+    return hipGetLastError();
+}
+
+bool gr_sop_fits(const GrGeom &g, const GrDevTables &t, long ltbl) {
+    (void)g;
+    return !t.dwtbl && ltbl >= 0 && ltbl <= 0xfff;
+}
+
+hipError_t gr_launch_sop_count(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, int n_heavy,
+                               uint4 *tile, hipStream_t s) {
+    const GridArgs A = make_args(g, t, n_ang, ltbl);
+    hipLaunchKernelGGL(k_sop_build<0>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, n_heavy, tile, (uint2 *)nullptr,
+                       (unsigned *)nullptr, (unsigned *)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t gr_launch_sop_fill(const GrGeom &g, const GrDevTables &t, int n_ang, long ltbl, int n_heavy,
+                              const uint4 *tile, uint2 *ch, unsigned *keys, unsigned *ent, hipStream_t s) {
+    const GridArgs A = make_args(g, t, n_ang, ltbl);
+    hipLaunchKernelGGL(k_sop_build<1>, dim3((A.ntiles + 3) / 4), dim3(256), 0, s, A, n_heavy, (uint4 *)tile, ch,
+                       keys, ent);
     return hipGetLastError();
 }
