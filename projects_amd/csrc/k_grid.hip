@@ -323,6 +323,12 @@ __device__ __forceinline__ unsigned wave_max(unsigned v) {
     return v;
 }
 
+__device__ __forceinline__ unsigned wave_min(unsigned v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, o));
+    return v;
+}
+
 // MODE 0 (count): rows of the ragged layout into tile_n, of the aligned one
 // into tile_na.  MODE 1 (fill): the entries at tile_off in the layout that
 // tile_n's top bit names (its rows in the low bits).
@@ -431,7 +437,8 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
 //
 // A tile's taps are cut into chunks at k_grid's merge points (every
 // kMergeAngles sample lines, and the last line) and wherever the next run of
-// samples would overflow the chunk's kSopSlots LDS slots.  Per chunk: a
+// samples (of one sample line, from the first to the last sample some lane
+// taps) would overflow the chunk's kSopSlots LDS slots.  Per chunk: a
 // header {rows | merge bit, slots}, its slots' records (kSopSlots keys
 // idx | mirror << 30 as the operator's entries, the unused ones zero; slot
 // 0 is record 0 of spec1, which K1 writes as zero), and its rows: per lane
@@ -495,13 +502,28 @@ __global__ void __launch_bounds__(256) k_sop_build(GridArgs A, int n_heavy, uint
     walk_tile(
         A, tx, ty, valid, pa - M2, pb - M2,
         [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
-            if (slots + (unsigned)nj > (unsigned)kSopSlots) close(false);
-            const unsigned sb = slots - (unsigned)c0;  // slot of sample j: sb + j
-            if (MODE == 1 && lane < nj)
-                keys[(unsigned long)chunk_index() * kSopSlots + slots + lane] =
-                    (unsigned)(row_off + c0 + lane) | (mirror ? kGopMirror : 0u);
-            slots += (unsigned)nj;
+            // stage only the samples some lane taps: [lo, hi] of the run
             const float qa = mirror ? ma : pa, qb = mirror ? mb : pb;
+            unsigned lo = 0xffffffffu, hi = 0u;
+            for (int j = j0; j <= j1; j++) {
+                int iu, iv;
+                if (gr_tap_index(j, C, S, qa, qb, A.g, iu, iv)) {
+                    lo = min(lo, (unsigned)j);
+                    hi = max(hi, (unsigned)j);
+                }
+            }
+            lo = wave_min(lo);
+            hi = wave_max(hi);
+            if (hi < lo) return;  // no lane taps this run (j >= 1, so hi = 0 means none)
+            (void)c0;
+            (void)nj;
+            const unsigned n = hi - lo + 1u;  // <= nj <= 64
+            if (slots + n > (unsigned)kSopSlots) close(false);
+            const unsigned sb = slots - lo;  // slot of sample j: sb + j
+            if (MODE == 1 && (unsigned)lane < n)
+                keys[(unsigned long)chunk_index() * kSopSlots + slots + lane] =
+                    (unsigned)(row_off + lo + lane) | (mirror ? kGopMirror : 0u);
+            slots += n;
             for (int j = j0; j <= j1; j++) {
                 int iu, iv;
                 if (gr_tap_index(j, C, S, qa, qb, A.g, iu, iv)) {
@@ -710,6 +732,7 @@ __device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs 
     Next nx = fetch(slot0);
     for (int slot = slot0; slot < A.sector_len; slot += stride) {
         const Next cur = nx;
+        const unsigned cur_ti = (unsigned)(first + slot);
         nx = fetch(min(slot + stride, A.sector_len - 1));
         if (cur.tt == kGridNoTile) continue;
         const int tx = (int)(cur.tt & 0xffffu), ty = (int)(cur.tt >> 16);
@@ -747,7 +770,7 @@ __device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs 
             sop_stage<P>(k0, base1, base2, stg, 0);
             if (ns > 64u) sop_stage<P>(k1, base1, base2, stg, 64);
             // the next chunk (or this one again past the last)
-            const unsigned cn = c + 1 < nch ? extra + c : (c ? extra + c - 1 : (unsigned)(first + slot));
+            const unsigned cn = c + 1 < nch ? extra + c : (c ? extra + c - 1 : cur_ti);
             const uint2 hn = O.sop_ch[cn];
             const unsigned kn0 = O.sop_keys[(unsigned long)cn * kSopSlots + lane];
             const unsigned kn1 = O.sop_keys[(unsigned long)cn * kSopSlots + 64 + lane];
@@ -977,6 +1000,7 @@ hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_ba
         const int sblocks = heavy_blocks(O->n_hitems) + kGridSectors * lb;
         GopArgs o = *O;
         o.sop_waves = 4 * lb;
+
         const size_t lds = lds_base + sizeof(float2) * 4 * kSopSlots * P;
         hipError_t e = gr_allow_lds(k_grid_op<ACCUM, P, true>, lds);
         if (e != hipSuccess) return e;
