@@ -157,8 +157,9 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     hipStream_t s0 = q->streams[0];
 
     // gather operators: a queue is built once and replayed, so a dataset's
-    // operator pays off from the first replay on.  Within a budget -- half of
-    // the device memory left after a generous estimate of the workspaces --
+    // operator pays off from the first replay on.  Within a budget -- three
+    // quarters of the device memory left after a generous estimate of the
+    // workspaces --
     // datasets are served in order of slices (the operator's benefit per byte
     // grows with the pairs it serves), then of size.
     {
@@ -168,7 +169,7 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         for (int d = 0; d < n_datasets; d++)
             ws_est = std::max(ws_est, gr_plan_workspace_bytes(q->plans[d], std::min((q->ds[d].n_slices + 1) / 2, 8)));
         ws_est *= (size_t)(n_streams + 1);
-        size_t budget = free_b > ws_est ? (free_b - ws_est) / 2 : 0;
+        size_t budget = free_b > ws_est ? (free_b - ws_est) / 4 * 3 : 0;
         std::vector<int> order(n_datasets);
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
