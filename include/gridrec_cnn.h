@@ -42,8 +42,10 @@ size_t gr_cnn_work_bytes(int n, int H, int W);
  * after every change of params. */
 int gr_cnn_pack(const float *params, unsigned short *packed, void *stream);
 
-/* Forward pass.  a1..a4 receive the activations (for inference a3 may alias
- * a1 and a4 a2).  y (may be NULL): the corrected images in the input's units;
+/* Forward pass.  a1..a4 receive the activations.  For inference a3 may alias
+ * a1 and a4 a2: then the first layer runs fused into the second, a1 is never
+ * written (a1's buffer ends holding a3, a2's a4), and y, yn and slice_stats
+ * are bitwise those of distinct buffers.  y (may be NULL): the corrected images in the input's units;
  * yn (may be NULL): the same in network units (x/scale + residual), needed by
  * gr_cnn_backward.  slice_stats (may be NULL, needs work): per image, the sums
  * of residual^2 and (x/scale)^2 in network units (2n floats) -- the

@@ -237,6 +237,132 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Inference with the first layer fused into the second (a1 never reaches
+// HBM): the 1 -> 32 layer runs on the tile grown by one pixel -- RR x RC =
+// 10 x 34 pixels, held as 10 rows of three 16-pixel groups (the last partly
+// idle) -- into LDS, zeros outside the image (the padding the second layer's
+// staging gives), then the second layer on the 8 x 32 tile from there.
+// Every value is computed by the same MFMA sequence and rounding as in the
+// layer-by-layer kernels: bitwise.  (The last layer fused into the fourth the
+// same way was slower: recomputing the 32 -> 32 layer on the grown tile costs
+// 1.9x its MFMA work and the 62 KB workgroup halved the occupancy.)
+constexpr int RR = TH + 2, RC = TW + 2;      // the grown tile (= HS x WS)
+constexpr int XR2 = TH + 4, XC2 = TW + 4;    // its input (x) region, two-pixel halo
+
+// Store lane's four channels co0..co0+3 of grown-tile pixel (r, c) as bf16 of
+// relu(acc + bias) -- or zeros for a pixel outside the image.
+__device__ __forceinline__ void put_grown(uint16_t *dst, int r, int c, int co0, floatx4 a, const float *bias,
+                                          bool inside) {
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const float z = a[e] + bias[co0 + e];
+        v[e] = inside && z > 0.0f ? z : 0.0f;
+    }
+    uint2 o;
+    o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+    o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+    *reinterpret_cast<uint2 *>(dst + (r * RC + c) * PSTR + co0) = o;
+}
+
+// Layers 1 and 2: a2 = layer2(layer1(x)) on 8 x 32 tiles, persistent as
+// k_conv32: bf16(x/scale) with a two-pixel halo in LDS, a1 on the grown tile
+// in LDS (k_conv_in's arithmetic), then k_conv32<0>'s.
+__global__ void __launch_bounds__(256) k_conv12(const float *__restrict__ x, float scale,
+                                                const float *__restrict__ prm, const uint16_t *__restrict__ wp,
+                                                uint16_t *__restrict__ y, int H, int W, int ntiles, TileMap tm) {
+    __shared__ float sxs[XR2 * XC2];
+    __shared__ __align__(16) uint16_t sx[RR * RC * PSTR];
+    __shared__ float sw[9 * CH + 2 * CH];  // W1 (bf16 values), b1, b2
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W1 + i]);
+    if (tid < CH) {
+        sw[9 * CH + tid] = prm[OFF_B1 + tid];
+        sw[10 * CH + tid] = prm[OFF_B2 + tid];
+    }
+    __syncthreads();
+    bf16x8 A1[2];
+    conv_in_frag(sw, lane, A1);
+    bf16x8 A[2][9];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int t = 0; t < 9; t++)
+            A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
+    const int j = lane & 15, g = lane >> 4;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();  // the previous tile's reads of sxs and sx are done
+        {
+            constexpr int PER = (XR2 * XC2 + 255) / 256;
+            float v[PER];
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int i = tid + k * 256;
+                const int r = i / XC2, c = i - r * XC2, gy = y0 + r - 2, gx = x0 + c - 2;
+                v[k] = i < XR2 * XC2 && gy >= 0 && gy < H && gx >= 0 && gx < W ? x[ibase + (long)gy * W + gx] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < PER; k++) {
+                const int i = tid + k * 256;
+                const int r = i / XC2, c = i - r * XC2, gy = y0 + r - 2, gx = x0 + c - 2;
+                if (i < XR2 * XC2) sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(v[k] / scale) : 0.0f;
+            }
+        }
+        __syncthreads();
+        // a1 on the grown tile: 30 groups of 16 pixels over the 4 waves
+#pragma unroll 1
+        for (int q = wave; q < RR * 3; q += 4) {
+            const int r = q / 3, c = (q % 3) * 16 + j, cc = c < RC ? c : RC - 1;
+            const bf16x8 B = conv_in_b(sxs, XC2, r, cc, lane);
+            const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[0], B, z, 0, 0, 0);
+            const floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[1], B, z, 0, 0, 0);
+            if (c < RC) {
+                const int gy = y0 + r - 1, gx = x0 + c - 1;
+                const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
+                put_grown(sx, r, c, 4 * g, acc0, sw + 9 * CH, inside);
+                put_grown(sx, r, c, 16 + 4 * g, acc1, sw + 9 * CH, inside);
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int q = 0; q < 4; q++) {
+            const int r = wave * 2 + (q >> 1), cb = (q & 1) * 16;
+            floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int ky = t / 3, kx = t - ky * 3;
+                const bf16x8 B = *reinterpret_cast<const bf16x8 *>(sx + ((r + ky) * WS + cb + j + kx) * PSTR + 8 * g);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0][t], B, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1][t], B, acc1, 0, 0, 0);
+            }
+            const int gy = y0 + r, gx = x0 + cb + j;
+            if (gy < H && gx < W) {
+                const long p = ibase + (long)gy * W + gx;
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++) {
+                    const floatx4 a = nb ? acc1 : acc0;
+                    const int co0 = nb * 16 + 4 * g;
+                    float v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const float zz = a[e] + sw[10 * CH + co0 + e];
+                        v[e] = zz > 0.0f ? zz : 0.0f;
+                    }
+                    uint2 o;
+                    o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                    o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                    *reinterpret_cast<uint2 *>(y + p * CH + co0) = o;
+                }
+            }
+        }
+    }
+}
+
 // Weight gradient of a 32 -> 32 layer: partial[block] = (dW[t][co][ci], db[co])
 // over the tiles blockIdx.x, blockIdx.x + gridDim.x, ...
 constexpr int DZ_CO = TH * TW + 8;     // sdz channel stride (bf16)
@@ -1044,12 +1170,23 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     const long npx = (long)n * H * W;
     const uint16_t *pk = reinterpret_cast<const uint16_t *>(packed);
     const int nconv = ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS;
-    hipLaunchKernelGGL(k_conv_in, dim3(nconv), dim3(256), 0, s, x, scale, params, a1, H, W, ntiles, tm);
-    uint16_t *acts[4] = {a1, a2, a3, a4};
-    const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
-    for (int l = 0; l < 3; l++)
-        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
-                           params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, ntiles, tm);
+    // inference (a3 aliasing a1 and a4 aliasing a2: the caller keeps no
+    // activation): the first layer fused into the second
+    const bool fused = a3 == a1 && a4 == a2;
+    if (fused) {
+        hipLaunchKernelGGL(k_conv12, dim3(nconv), dim3(256), 0, s, x, scale, params, pk, a2, H, W, ntiles, tm);
+        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, a2, pk + PACK_LAYER, params + OFF_B3,
+                           (const uint16_t *)nullptr, a3, H, W, ntiles, tm);
+        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, a3, pk + 2L * PACK_LAYER, params + OFF_B4,
+                           (const uint16_t *)nullptr, a4, H, W, ntiles, tm);
+    } else {
+        hipLaunchKernelGGL(k_conv_in, dim3(nconv), dim3(256), 0, s, x, scale, params, a1, H, W, ntiles, tm);
+        uint16_t *acts[4] = {a1, a2, a3, a4};
+        const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
+        for (int l = 0; l < 3; l++)
+            hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
+                               params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, ntiles, tm);
+    }
     float *stats = nullptr;
     if (slice_stats) {
         char *wb = (char *)work;

@@ -99,6 +99,22 @@ def test_fused_inference_is_bitwise_forward(shape):
     assert torch.allclose(s1, s2, rtol=1e-4)
 
 
+@pytest.mark.parametrize("shape", [(3, 37, 70), (1, 16, 32), (2, 5, 7), (1, 130, 200)])
+def test_inference_forward_is_bitwise_training_forward(shape):
+    """forward() for inference (activation buffers aliased: the first layer
+    fused into the second, a1 never stored) gives bitwise the outputs and
+    per-image sums of the training forward (five layer kernels, every
+    activation stored), borders and ragged tiles included."""
+    net, _, _, _ = make(6)
+    x = torch.randn(shape, device="cuda") * 2.0
+    y1, s1 = net.forward(x, train=True)
+    y1, s1 = y1.clone(), s1.clone()
+    y2, s2 = net.forward(x)
+    torch.cuda.synchronize()
+    assert torch.equal(y1, y2)
+    assert torch.equal(s1, s2)
+
+
 def test_backward_matches_oracle():
     net, pd, x, t = make(2)
     xg, tg = torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda()
