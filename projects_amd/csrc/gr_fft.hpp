@@ -35,6 +35,39 @@
 
 __host__ __device__ __forceinline__ int gr_lds_idx(int i) { return i ^ ((i >> 4) & 31); }
 
+// Raw buffer view of the `bytes` bytes at base (<= 2^31): a load at a byte
+// offset at or past the end returns zeros and touches no memory.  Loads with
+// a lane-dependent bound thus need no branch: every one of a thread's loads
+// is issued before the first wait (a guarded load whose result is copied
+// inside its branch waits for it there).
+struct GrBuf {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __amdgpu_buffer_rsrc_t r;
+#endif
+};
+constexpr unsigned kGrBufNone = 0x80000000u;  // an offset past every view's end
+__device__ __forceinline__ GrBuf gr_buf(const void *base, unsigned bytes) {
+    GrBuf b;
+#if defined(__HIP_DEVICE_COMPILE__)
+    b.r = __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+#else
+    (void)base;
+    (void)bytes;
+#endif
+    return b;
+}
+// soff: a wave-uniform part of the offset (a scalar register), added to off.
+__device__ __forceinline__ float2 gr_buf_ld2(GrBuf b, unsigned off, unsigned soff = 0) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)off, (int)soff, 0));
+#else
+    (void)b;
+    (void)off;
+    (void)soff;
+    return make_float2(0.0f, 0.0f);
+#endif
+}
+
 // Entries of the pass-ordered twiddles of an N-point transform whose passes
 // go up to max_radix (16 or 4).
 __host__ __device__ __forceinline__ long gr_twp_len(long N, int max_radix) {
@@ -104,20 +137,40 @@ __device__ __forceinline__ void gr_dft16(float2 (&v)[16]) {
 // No output hook: the transform ends in buf.
 struct GrFftToLds {
     static constexpr bool enabled = false;
-    __device__ void operator()(int, float2) const {}
+    __device__ void prep(int, int) {}
+    __device__ void operator()(int, float2, int) const {}
 };
 
+// The first radix-16 pass of butterfly j (Ns = 1, no twiddles) on values
+// already in registers, v[r] = x[j + r N/16], and its outputs stored to buf
+// where gr_lds_fft's own first pass puts them.
+template <int SIGN>
+__device__ __forceinline__ void gr_fft_first_pass16(float2 *buf, float2 (&v)[16], int j) {
+    gr_dft16<SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < 16; q++) buf[gr_lds_idx((j << 4) + q)] = v[4 * (q & 3) + (q >> 2)];
+}
+
 // Returns true when an enabled output hook `out` received the result instead
-// of buf: when the last pass is a radix-16 one, out(i, X[i]) is called for
+// of buf: when the last pass is a radix-16 one, out(i, X[i], s) is called for
 // every output i straight from the registers of that pass (no LDS store, no
-// barrier); otherwise the result is in buf as usual and false is returned.
-// Every output i is handed over exactly once; for a fixed pass slot the
-// threads hand over consecutive i.
-template <int SIGN, int MAXB, class Out = GrFftToLds>
+// barrier), s < 16 MAXB16 the output's slot among the thread's outputs; before
+// any of these calls, out.prep(i, s) is called for every one of them (the
+// hook issues the loads its outputs need there, all in flight together).
+// Otherwise the result is in buf as usual and false is returned.  Every output
+// i is handed over exactly once; for a fixed pass slot the threads hand over
+// consecutive i.  first_done: the caller ran the first radix-16 pass
+// (gr_fft_first_pass16 for every butterfly, then a barrier); only with
+// max_radix 16 and N >= 16.  TW_LDS: tw is a copy of the radix-16 part of
+// the table in LDS (only with max_radix 16 and N a power of 16).
+template <int SIGN, int MAXB, class Out = GrFftToLds, bool TW_LDS = false>
 __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__restrict__ tw, int tid, int nthr,
-                                           int max_radix = 16, Out out = Out()) {
-    int Ns = 1;
-    const float2 *__restrict__ twp = tw + (max_radix >= 16 ? 0 : gr_twp_len(N, 16));
+                                           int max_radix = 16, Out out = Out(), bool first_done = false) {
+    int Ns = first_done ? 16 : 1;  // the Ns = 1 pass reads no twiddles: tofs stays at the table's start
+    // twiddles by buffer loads: one lane offset (8 k) for all of a pass's
+    // loads, the entry's row in the scalar offset, tofs = the pass's table
+    const GrBuf twb = gr_buf(tw, (unsigned)((gr_twp_len(N, 16) + gr_twp_len(N, 4)) * sizeof(float2)));
+    unsigned tofs = max_radix >= 16 ? 0u : (unsigned)(gr_twp_len(N, 16) * sizeof(float2));
     constexpr int MAXB16 = (MAXB + 3) / 4;  // radix-16 butterflies per thread
     const int nb16 = N >> 4;
     while (max_radix >= 16 && Ns * 16 <= N) {
@@ -127,11 +180,21 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
             const int j = tid + c * nthr;
             if (j < nb16) {
                 const int k = j & (Ns - 1);
+                // the twiddles into registers of their own first: all fifteen
+                // loads in flight together (loaded next to their products the
+                // compiler reuses one register pair, one round trip each)
+                float2 w[15];
+                if (Ns > 1) {
+#pragma unroll
+                    for (int r = 1; r < 16; r++)
+                        w[r - 1] = TW_LDS ? tw[tofs / 8u + (unsigned)((r - 1) * Ns + k)]
+                                          : gr_buf_ld2(twb, 8u * (unsigned)k, tofs + 8u * (unsigned)((r - 1) * Ns));
+                }
 #pragma unroll
                 for (int r = 0; r < 16; r++) v[c][r] = buf[gr_lds_idx(j + r * nb16)];
                 if (Ns > 1) {
 #pragma unroll
-                    for (int r = 1; r < 16; r++) v[c][r] = gr_cmul(v[c][r], twp[(r - 1) * Ns + k]);
+                    for (int r = 1; r < 16; r++) v[c][r] = gr_cmul(v[c][r], w[r - 1]);
                 }
                 gr_dft16<SIGN>(v[c]);
             }
@@ -139,12 +202,19 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
         if (Out::enabled && Ns * 16 == N) {  // the last pass: outputs straight from registers
 #pragma unroll
             for (int c = 0; c < MAXB16; c++) {
+                const int j = tid + c * nthr, k = j & (Ns - 1);
+                const int base = ((j - k) << 4) + k;
+#pragma unroll
+                for (int q = 0; q < 16; q++) out.prep(j < nb16 ? base + q * Ns : 0, 16 * c + q);
+            }
+#pragma unroll
+            for (int c = 0; c < MAXB16; c++) {
                 const int j = tid + c * nthr;
                 if (j < nb16) {
                     const int k = j & (Ns - 1);
                     const int base = ((j - k) << 4) + k;
 #pragma unroll
-                    for (int q = 0; q < 16; q++) out(base + q * Ns, v[c][4 * (q & 3) + (q >> 2)]);
+                    for (int q = 0; q < 16; q++) out(base + q * Ns, v[c][4 * (q & 3) + (q >> 2)], 16 * c + q);
                 }
             }
             return true;
@@ -161,7 +231,7 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
             }
         }
         __syncthreads();
-        if (Ns > 1) twp += 15 * Ns;
+        if (Ns > 1) tofs += 8u * 15u * (unsigned)Ns;
         Ns <<= 4;
     }
     const int nb4 = N >> 2;
@@ -172,12 +242,16 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
             const int j = tid + c * nthr;
             if (j < nb4) {
                 const int k = j & (Ns - 1);
+                float2 w[3];
+                if (Ns > 1) {
+#pragma unroll
+                    for (int r = 0; r < 3; r++) w[r] = gr_buf_ld2(twb, 8u * (unsigned)k, tofs + 8u * (unsigned)(r * Ns));
+                }
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[c][r] = buf[gr_lds_idx(j + r * nb4)];
                 if (Ns > 1) {
-                    v[c][1] = gr_cmul(v[c][1], twp[k]);
-                    v[c][2] = gr_cmul(v[c][2], twp[Ns + k]);
-                    v[c][3] = gr_cmul(v[c][3], twp[2 * Ns + k]);
+#pragma unroll
+                    for (int r = 1; r < 4; r++) v[c][r] = gr_cmul(v[c][r], w[r - 1]);
                 }
                 gr_dft4<SIGN>(v[c][0], v[c][1], v[c][2], v[c][3]);
             }
@@ -194,7 +268,7 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
             }
         }
         __syncthreads();
-        if (Ns > 1) twp += 3 * Ns;
+        if (Ns > 1) tofs += 8u * 3u * (unsigned)Ns;
         Ns <<= 2;
     }
     if (Ns < N) {  // N = 2 Ns: one radix-2 pass, twiddle e^{SIGN 2 pi i k/N}
@@ -206,7 +280,7 @@ __device__ __forceinline__ bool gr_lds_fft(float2 *buf, int N, const float2 *__r
             if (j < nb2) {
                 const int k = j & (Ns - 1);
                 const float2 x0 = buf[gr_lds_idx(j)];
-                const float2 x1 = gr_cmul(buf[gr_lds_idx(j + nb2)], twp[k]);
+                const float2 x1 = gr_cmul(buf[gr_lds_idx(j + nb2)], gr_buf_ld2(twb, 8u * (unsigned)k, tofs));
                 a[c] = make_float2(x0.x + x1.x, x0.y + x1.y);
                 b[c] = make_float2(x0.x - x1.x, x0.y - x1.y);
             }

@@ -11,6 +11,9 @@
 // but keeps only the n outputs p = iu(j) of the window; K4 transforms those
 // n columns down b, keeps the n outputs q = iv(k) and writes the two images
 // row by row with the deapodization product fused in.
+#include <algorithm>
+#include <atomic>
+
 #include "gr_fft.hpp"
 #include "gr_gather.hpp"
 #include "gr_internal.h"
@@ -39,7 +42,8 @@ struct WindowRow {
     float2 *dst;
     int M, m02, off, out_n;
     static constexpr bool enabled = true;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
+    __device__ __forceinline__ void prep(int, int) {}
+    __device__ __forceinline__ void operator()(int i, float2 v, int) const {
         const int jp = i >= M - m02 ? i - (M - m02) : i + m02;
         const int jj = jp - off;
         if (jj >= 0 && jj < out_n) dst[jj] = v;
@@ -79,15 +83,18 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
     // when the last pass is radix 16, the window's outputs go to HBM straight
     // from its registers: output i is window column j' = corner_index^-1(i)
     if (gr_lds_fft<-1, MAXB>(row, M, A.tw, tid, nthr, A.radix, WindowRow{dst, M, m02, off, out_n})) return;
+    // an array of its own (reusing v would keep the loaded row alive, in
+    // registers, through the whole transform)
+    float2 o[EPT];
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
         const int jj = tid + r * nthr;
-        if (jj < out_n) v[r] = row[gr_lds_idx((int)corner_index(off + jj, M, m02))];
+        o[r] = row[gr_lds_idx((int)corner_index(jj < out_n ? off + jj : off, M, m02))];
     }
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
         const int jj = tid + r * nthr;
-        if (jj < out_n) dst[jj] = v[r];
+        if (jj < out_n) dst[jj] = o[r];
     }
 }
 
@@ -104,18 +111,23 @@ struct ColsArgs {
 // Output hook of the column transform: FFT output i is image column
 // kk = k' - out_off of this row, corner_index(k') = i, scaled by the
 // deapodization cu * winv[k'] (grid.cpp:401); rows past the window have
-// r1 = nullptr and write nothing.
+// r1 = nullptr and write nothing.  prep loads winv[k'] of every output of the
+// thread (k' < M always: no bounds test, so the loads are issued together
+// rather than each behind the previous output's stores).  NS: output slots.
+template <int NS>
 struct WindowCol {
     float *r1, *r2;
     const float *winv;
     float cu;
     int M, m02, off, out_n;
+    float wv[NS];
     static constexpr bool enabled = true;
-    __device__ __forceinline__ void operator()(int i, float2 v) const {
-        const int kp = i >= M - m02 ? i - (M - m02) : i + m02;
-        const int kk = kp - off;
+    __device__ __forceinline__ int kprime(int i) const { return i >= M - m02 ? i - (M - m02) : i + m02; }
+    __device__ __forceinline__ void prep(int i, int s) { wv[s] = winv[kprime(i)]; }
+    __device__ __forceinline__ void operator()(int i, float2 v, int s) const {
+        const int kk = kprime(i) - off;
         if (r1 && kk >= 0 && kk < out_n) {
-            const float corrn = cu * winv[kp];
+            const float corrn = cu * wv[s];
             r1[kk] = corrn * v.x;
             if (r2) r2[kk] = corrn * v.y;
         }
@@ -151,14 +163,17 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
     const int out_n = (int)A.out_n;
     constexpr int EPT = 4 * MAXB;  // rows per thread and column: M <= 4 MAXB nthr
     if (CW) {
-        // unrolled, every load before the first LDS store; the CW columns of
-        // a row come from neighbouring lanes
+        // unrolled, every load before the first LDS store (buffer loads: a
+        // column past the window reads zeros without a branch); the CW
+        // columns of a row come from neighbouring lanes
+        const GrBuf zb = gr_buf(zp, (unsigned)((long)M * out_n * sizeof(float2)));
         float2 v[EPT];
 #pragma unroll
         for (int r = 0; r < EPT; r++) {
             const int idx = (int)threadIdx.x + r * (int)blockDim.x;
             const int b = idx / CW, c = idx - b * CW;
-            v[r] = b < M && c0 + c < out_n ? zp[(long)b * out_n + c0 + c] : make_float2(0.0f, 0.0f);
+            const bool in = b < M && c0 + c < out_n;
+            v[r] = gr_buf_ld2(zb, in ? (unsigned)(((long)b * out_n + c0 + c) * sizeof(float2)) : kGrBufNone);
         }
 #pragma unroll
         for (int r = 0; r < EPT; r++) {
@@ -185,9 +200,10 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
         // straight from that pass's registers (deapodization on the way)
         const long jj = c0 + grp;
         const bool live = jj < A.out_n;
-        WindowCol wc{live ? o1 + jj * A.out_row_stride : nullptr,
-                     live && o2 ? o2 + jj * A.out_row_stride : nullptr,
-                     A.winv, live ? A.winv[A.out_off + jj] : 0.0f, M, (int)A.M02, (int)A.out_off, out_n};
+        WindowCol<16 * ((MAXB + 3) / 4)> wc{live ? o1 + jj * A.out_row_stride : nullptr,
+                                            live && o2 ? o2 + jj * A.out_row_stride : nullptr,
+                                            A.winv, live ? A.winv[A.out_off + jj] : 0.0f, M, (int)A.M02,
+                                            (int)A.out_off, out_n, {}};
         if (gr_lds_fft<-1, MAXB>(cols + grp * gr_lds_len(M), M, A.tw, tid, nthr, A.radix, wc)) return;
     } else {
         for (int c = 0; c < cw; c++) gr_lds_fft<-1, MAXB>(cols + c * gr_lds_len(M), M, A.tw, tid, nthr, A.radix);
@@ -229,6 +245,96 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
             r1[kk] = corrn * v.x;
             if (r2) r2[kk] = corrn * v.y;
         }
+    }
+}
+
+// Persistent column transform for M = 4096 with 4 columns per item and
+// radix-16 passes (the plain 2048-detector geometry): the columns and the
+// 32 KiB of twiddles fill the CU's 160 KiB of LDS, so one workgroup of 1024
+// threads runs per CU and loops over (pair, column group) items.  Each thread
+// loads rows j, j + 256, ..., j + 3840 of one column -- exactly the inputs of
+// the first radix-16 butterfly j -- so that pass runs from the registers that
+// received the loads; the next item's loads are issued into the same
+// registers right after, and stay in flight while passes 2 and 3 run on
+// twiddles from LDS (no global load in between: the loads of all of a wave
+// complete in order, so any would wait for them).  The last pass ends in LDS
+// and an epilogue scales and stores the image row (an output hook would need
+// sixteen more registers for its deapodization factors: spills).  Same
+// arithmetic, twiddles and output values as k_ifft_cols: bitwise its images.
+// Needs 1 <= out_n <= 2048.
+constexpr int kPfM = 4096, kPfCw = 4, kPfThreads = 1024;
+constexpr int kPfNthr = kPfThreads / kPfCw;  // threads per column
+constexpr int kPfColStride = kPfM + 4;       // padded: the four columns' first-pass stores use different banks
+constexpr int kPfTw = 15 * 16 + 15 * 256;    // pass-ordered radix-16 twiddles of 4096 points
+constexpr size_t kPfLds = sizeof(float2) * ((size_t)kPfCw * kPfColStride + kPfTw);
+
+__global__ void __launch_bounds__(kPfThreads) k_ifft_cols_pf(ColsArgs A, int n_items) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds + kPfCw * kPfColStride;
+    const int out_n = (int)A.out_n, off = (int)A.out_off, m02 = (int)A.M02;
+    const int ngroups = (out_n + kPfCw - 1) / kPfCw;
+    for (int i = (int)threadIdx.x; i < kPfTw; i += kPfThreads) tw[i] = A.tw[i];
+    float2 v[16];
+    float cu_next = 0.0f;
+    // the item's column loads: thread t reads rows t/4 + 256 r of column t % 4
+    // (the inputs of first-pass butterfly t/4); a column past the window (the
+    // last group only) reads zeros through the buffer bound, no branch
+    auto load = [&](int item) {
+        const int t = (int)threadIdx.x, lc = t % kPfCw;
+        const int p = item / ngroups, x = item - p * ngroups;
+        const int c0 = (int)column_group((unsigned)x, (unsigned)ngroups) * kPfCw;
+        const GrBuf zb = gr_buf(A.z + (long)p * kPfM * out_n, (unsigned)((long)kPfM * out_n * sizeof(float2)));
+        const bool in = c0 + lc < out_n;
+        const unsigned o0 = in ? (unsigned)(((t / kPfCw) * out_n + c0 + lc) * sizeof(float2)) : kGrBufNone;
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            v[r] = gr_buf_ld2(zb, o0, (unsigned)(r * kPfNthr * out_n * sizeof(float2)));
+        const int grp = t / kPfNthr;
+        cu_next = A.winv[off + (c0 + grp < out_n ? c0 + grp : out_n - 1)];
+    };
+    int item = (int)blockIdx.x;
+    if (item < n_items) load(item);
+    __syncthreads();  // twiddles in place
+    while (item < n_items) {  // the same trip count in every thread of the block
+        // thread indices recomputed from an opaque copy each iteration: held
+        // from outside the loop, their derived addresses would not fit the
+        // 128 registers of a 1024-thread workgroup
+        int t = (int)threadIdx.x;
+        asm volatile("" : "+v"(t));
+        gr_fft_first_pass16<-1>(lds + (t % kPfCw) * kPfColStride, v, t / kPfCw);
+        const int cur = item;
+        const float cu = cu_next;
+        item += (int)gridDim.x;
+        __syncthreads();
+        if (item < n_items) load(item);
+        const int grp = t / kPfNthr, tid = t % kPfNthr;
+        const int p = cur / ngroups, x = cur - p * ngroups;
+        const long jj = column_group((unsigned)x, (unsigned)ngroups) * kPfCw + grp;
+        const long s1 = 2L * (A.pair0 + p), s2 = s1 + 1;
+        const bool live = jj < out_n;
+        float *r1 = live ? A.out + s1 * A.out_slice_stride + jj * A.out_row_stride : nullptr;
+        float *r2 = live && s2 < A.n_slices ? A.out + s2 * A.out_slice_stride + jj * A.out_row_stride : nullptr;
+        float2 *col = lds + grp * kPfColStride;
+        gr_lds_fft<-1, 4, GrFftToLds, true>(col, kPfM, tw, tid, kPfNthr, 16, GrFftToLds(), true);
+        // image row jj: output kk = tid + 256 r (r < 8 covers a window of up to 2048)
+        float wk[8];
+        float2 o[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int kk = tid + r * kPfNthr, k = off + (kk < out_n ? kk : 0);
+            wk[r] = A.winv[k];
+            o[r] = col[gr_lds_idx(k < m02 ? kPfM - m02 + k : k - m02)];
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const int kk = tid + r * kPfNthr;
+            if (r1 && kk < out_n) {
+                const float corrn = cu * wk[r];  // grid.cpp:401
+                r1[kk] = corrn * o[r].x;
+                if (r2) r2[kk] = corrn * o[r].y;
+            }
+        }
+        __syncthreads();  // the columns are free for the next item's first pass
     }
 }
 
@@ -297,6 +403,32 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
 // MI355X: 0.57 ms per launch of 8 pairs with 1, 0.50 with 2, 0.48 with 4).
 // Callers may ask for a width (gr_plan.cols_per_block).
 
+// A figure of the current device, queried once per device (0 when the query
+// fails).
+static int device_attr(hipDeviceAttribute_t attr, std::atomic<int> (&cache)[64]) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = -1;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v > 0 ? v : 0;
+}
+
+static int gr_cu_count() {
+    static std::atomic<int> cache[64];
+    return device_attr(hipDeviceAttributeMultiprocessorCount, cache);
+}
+
+// The persistent column transform needs the whole 160 KiB of a CU's LDS.
+static bool pf_fits() {
+    static std::atomic<int> optin[64], plain[64];
+    const int lds = std::max(device_attr(hipDeviceAttributeSharedMemPerBlockOptin, optin),
+                             device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, plain));
+    return gr_cu_count() > 0 && (size_t)lds >= kPfLds;
+}
+
 static bool cols_side_by_side(long M, int cw) {
     return cw >= 1 && cw <= 8 && (cw & (cw - 1)) == 0 && (long)cw * gr_fft_threads(M) <= 1024 &&
            (long)cw * M * 8 <= 163840;
@@ -332,6 +464,15 @@ hipError_t gr_launch_ifft_cols(const GrGeom &g, const GrDevTables &t, const floa
     A.out_n = g.out_n;
     const int nthr = gr_fft_threads(g.M);
     const bool side = cols_side_by_side(g.M, A.cw);
+    if (side && A.cw == kPfCw && g.M == kPfM && nthr == kPfNthr && g.fft_radix == 16 && g.out_n <= 8 * kPfNthr &&
+        pf_fits()) {
+        const int n_items = (int)((g.out_n + kPfCw - 1) / kPfCw) * pairs;
+        const int nblk = std::min(n_items, gr_cu_count());
+        hipError_t e = gr_allow_lds(k_ifft_cols_pf, kPfLds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_ifft_cols_pf, dim3(nblk), dim3(kPfThreads), kPfLds, s, A, n_items);
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)((g.out_n + A.cw - 1) / A.cw), pairs), block(side ? A.cw * nthr : nthr);
     const size_t lds = sizeof(float2) * A.cw * gr_lds_len(g.M);
 #define GR_COLS_K(MB, CW)                                                                \
