@@ -56,6 +56,26 @@ __device__ __forceinline__ GrBuf gr_buf(const void *base, unsigned bytes) {
 #endif
     return b;
 }
+// Four bytes at byte offset off (zero past the end).
+__device__ __forceinline__ float gr_buf_ldf(GrBuf b, unsigned off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)off, 0, 0));
+#else
+    (void)b;
+    (void)off;
+    return 0.0f;
+#endif
+}
+// Sixteen bytes at byte offset off (zeros past the end).
+__device__ __forceinline__ uint4 gr_buf_ld16(GrBuf b, unsigned off) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(b.r, (int)off, 0, 0));
+#else
+    (void)b;
+    (void)off;
+    return make_uint4(0u, 0u, 0u, 0u);
+#endif
+}
 // soff: a wave-uniform part of the offset (a scalar register), added to off.
 __device__ __forceinline__ float2 gr_buf_ld2(GrBuf b, unsigned off, unsigned soff = 0) {
 #if defined(__HIP_DEVICE_COMPILE__)

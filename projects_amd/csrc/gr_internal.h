@@ -83,6 +83,10 @@ struct GrDevTables {
     const int2 *row_tiles = nullptr;
 };
 
+// Compute units of the current device (queried once per device; 0 when the
+// query fails).
+int gr_cu_count();
+
 // ---- kernel launchers (one per .hip file) ----------------------------------
 
 // The spectra of a chunk's pairs are interleaved by groups of kSpecGroup

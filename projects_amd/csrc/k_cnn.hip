@@ -168,13 +168,44 @@ struct TileMap {
     }
 };
 
+// A tile's activations with their halo (stage_tile's pieces) loaded into
+// registers through a buffer bounded to the tile's image: pieces outside the
+// image read zeros with no branch, so all six loads are in flight together
+// and none is waited for until the pieces are stored (needs H W 64 < 2^31).
+constexpr int STAGE_PER = (HS * WS * 4 + 255) / 256;
+__device__ __forceinline__ void prefetch_tile(const uint16_t *__restrict__ x, int H, int W, int img, int y0, int x0,
+                                              int tid, uint4 (&v)[STAGE_PER]) {
+    const GrBuf b = gr_buf(x + (long)img * H * W * CH, (unsigned)((long)H * W * CH * sizeof(uint16_t)));
+#pragma unroll
+    for (int k = 0; k < STAGE_PER; k++) {
+        const int i = tid + k * 256;
+        const int pix = i >> 2, ch = i & 3;
+        const int r = pix / WS, c = pix - r * WS;
+        const int gy = y0 + r - 1, gx = x0 + c - 1;
+        const bool in = i < HS * WS * 4 && gy >= 0 && gy < H && gx >= 0 && gx < W;
+        v[k] = gr_buf_ld16(b, in ? (unsigned)(((gy * W + gx) * CH + ch * 8) * sizeof(uint16_t)) : kGrBufNone);
+    }
+}
+__device__ __forceinline__ void put_tile(uint16_t *sx, int tid, const uint4 (&v)[STAGE_PER]) {
+#pragma unroll
+    for (int k = 0; k < STAGE_PER; k++) {
+        const int i = tid + k * 256;
+        if (i < HS * WS * 4) *reinterpret_cast<uint4 *>(sx + (i >> 2) * PSTR + (i & 3) * 8) = v[k];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 32 -> 32 convolution on MFMA.  MODE 0: y = bf16(relu(acc + bias));
-// MODE 1 (input gradient): y = bf16(acc * (mask > 0)).
-template <int MODE>
+// MODE 1 (input gradient): y = bf16(acc * (mask > 0)).  PF (MODE 0, an
+// image under 2^31 bytes): the next tile's activations are loaded into
+// registers while this tile's MFMAs run (the bias is held in registers, so
+// no global load sits between them and the wait); same values, same
+// arithmetic.
+template <int MODE, bool PF = false>
 __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, const uint16_t *__restrict__ wp,
                                                 const float *__restrict__ bias, const uint16_t *__restrict__ mask,
                                                 uint16_t *__restrict__ y, int H, int W, int ntiles, TileMap tm) {
+    static_assert(!PF || MODE == 0, "prefetch: forward only");
     __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // weight fragments, loaded once per wave for all its tiles: lane holds
@@ -185,13 +216,38 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
 #pragma unroll
         for (int t = 0; t < 9; t++)
             A[nb][t] = *reinterpret_cast<const bf16x8 *>(wp + ((t * CH + nb * 16 + (lane & 15)) * CH + 8 * (lane >> 4)));
+    float bz[2][4];
+    if (PF) {
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) bz[nb][e] = bias[nb * 16 + 4 * (lane >> 4) + e];
+        if ((int)blockIdx.x < ntiles) {
+            int img, y0, x0;
+            tm.of(blockIdx.x, img, y0, x0);
+            uint4 pv[STAGE_PER];
+            prefetch_tile(x, H, W, img, y0, x0, tid, pv);
+            put_tile(sx, tid, pv);
+        }
+    }
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     int img, y0, x0;
     tm.of(tile, img, y0, x0);
     const long ibase = (long)img * H * W;
-    __syncthreads();  // the previous tile's reads of sx are done
-    stage_tile<3>(x, sx, ibase, H, W, y0, x0, tid);
-    __syncthreads();
+    uint4 pv[STAGE_PER];
+    const int next = tile + (int)gridDim.x;
+    if (PF) {
+        __syncthreads();  // this tile's pieces are in sx
+        if (next < ntiles) {
+            int ni, ny, nx;
+            tm.of(next, ni, ny, nx);
+            prefetch_tile(x, H, W, ni, ny, nx, tid, pv);
+        }
+    } else {
+        __syncthreads();  // the previous tile's reads of sx are done
+        stage_tile<3>(x, sx, ibase, H, W, y0, x0, tid);
+        __syncthreads();
+    }
 
     const int j = lane & 15, g = lane >> 4;
 #pragma unroll 1
@@ -217,7 +273,7 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
                 if (MODE == 0) {
 #pragma unroll
                     for (int e = 0; e < 4; e++) {
-                        const float z = v[e] + bias[co0 + e];
+                        const float z = v[e] + (PF ? bz[nb][e] : bias[co0 + e]);
                         v[e] = z > 0.0f ? z : 0.0f;
                     }
                 } else {
@@ -233,6 +289,10 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
                 *reinterpret_cast<uint2 *>(y + p * CH + co0) = o;
             }
         }
+    }
+    if (PF && next < ntiles) {
+        __syncthreads();  // this tile's reads of sx are done
+        put_tile(sx, tid, pv);
     }
   }
 }
@@ -644,54 +704,115 @@ __global__ void __launch_bounds__(256) k_conv_in_wgrad(const float *__restrict__
 // ---------------------------------------------------------------------------
 // 32 -> 1 layer with the residual: r = W5 * a4 + b5, yn = x/scale + r,
 // y = yn * scale; per-tile sums of r^2 and (x/scale)^2 for the fallback test.
-// One 8 x 32 tile per block: a4 with a one-pixel halo staged in LDS, each wave
-// 4 groups of 16 pixels on MFMA (conv_out_mfma), lanes 0..15 finishing them.
+// An 8 x 32 tile at a time: a4 with a one-pixel halo staged in LDS, each wave
+// 4 groups of 16 pixels on MFMA (conv_out_mfma), lanes 0..15 finishing them;
+// the tile's sums by a pairwise tree (sred[i] += sred[i + h], h = 128 .. 1:
+// the two steps across waves through LDS, the rest inside wave 0 by lane
+// shuffles, the same additions in the same order).  PF (an image under 2^31
+// bytes of activations): persistent workgroups, the next tile's a4 pieces and
+// x values loaded into registers while this tile is computed; otherwise one
+// tile per workgroup.  The same values either way.
+template <bool PF>
 __global__ void __launch_bounds__(256) k_conv_out(const uint16_t *__restrict__ a4, const float *__restrict__ x,
                                                   float scale, const float *__restrict__ prm, float *__restrict__ y,
                                                   float *__restrict__ yn, float *__restrict__ stats, int H, int W,
-                                                  TileMap tm) {
+                                                  int ntiles, TileMap tm) {
     __shared__ __align__(16) uint16_t sx[HS * WS * PSTR];
     __shared__ float sw[9 * CH + 1];
     __shared__ float sred[2][TH * TW];
-    int img, y0, x0;
-    tm.of(blockIdx.x, img, y0, x0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const long ibase = (long)img * H * W;
+    const int j = lane & 15;
     for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W5 + i]);
     if (tid == 0) sw[9 * CH] = prm[OFF_B5];
-    stage_tile<6>(a4, sx, ibase, H, W, y0, x0, tid);
-    for (int i = tid; i < TH * TW; i += 256) sred[0][i] = sred[1][i] = 0.0f;
+    // x of the lane's four pixels (lanes 0..15; zero outside the image)
+    auto load_x = [&](int img, int y0, int x0, float (&xv)[4]) {
+        const GrBuf b = gr_buf(x + (long)img * H * W, (unsigned)((long)H * W * sizeof(float)));
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int gy = y0 + wave * 2 + (q >> 1), gx = x0 + (q & 1) * 16 + j;
+            const bool in = lane < 16 && gy < H && gx < W;
+            xv[q] = gr_buf_ldf(b, in ? (unsigned)((gy * W + gx) * sizeof(float)) : kGrBufNone);
+        }
+    };
+    float xv[4];
+    int tile = PF ? (int)blockIdx.x : 0;
+    if (PF) {
+        if (tile < ntiles) {
+            int img, y0, x0;
+            tm.of(tile, img, y0, x0);
+            uint4 pv[STAGE_PER];
+            prefetch_tile(a4, H, W, img, y0, x0, tid, pv);
+            load_x(img, y0, x0, xv);
+            put_tile(sx, tid, pv);
+        }
+    } else {
+        tile = (int)blockIdx.x;
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        stage_tile<6>(a4, sx, (long)img * H * W, H, W, y0, x0, tid);
+    }
     __syncthreads();
     bf16x8 A[9];
     conv_out_frag(sw, lane, A);
-    const int j = lane & 15;
-#pragma unroll 1
-    for (int q = 0; q < 4; q++) {
-        const int r = wave * 2 + (q >> 1), c = (q & 1) * 16 + j;
-        const floatx4 acc = conv_out_mfma(sx, WS, r, c, lane, A);
-        const int gy = y0 + r, gx = x0 + c;
-        if (lane < 16 && gy < H && gx < W) {
-            const float s = acc[0] + sw[9 * CH];
-            const long p = ibase + (long)gy * W + gx;
-            const float xn = x[p] / scale;
-            const float v = xn + s;
-            if (yn) yn[p] = v;
-            if (y) y[p] = v * scale;
-            sred[0][r * TW + c] = s * s;
-            sred[1][r * TW + c] = xn * xn;
+    const float b5 = sw[9 * CH];
+    for (; tile < ntiles; tile += PF ? (int)gridDim.x : ntiles) {
+        int img, y0, x0;
+        tm.of(tile, img, y0, x0);
+        const long ibase = (long)img * H * W;
+        __syncthreads();  // this tile's pieces in sx; the previous tile's sums read
+        const int next = tile + (int)gridDim.x;
+        uint4 pv[STAGE_PER];
+        float xn_next[4];
+        if (PF && next < ntiles) {
+            int ni, ny, nx;
+            tm.of(next, ni, ny, nx);
+            prefetch_tile(a4, H, W, ni, ny, nx, tid, pv);
+            load_x(ni, ny, nx, xn_next);
         }
-    }
-    __syncthreads();
-    for (int h = TH * TW / 2; h > 0; h >>= 1) {
-        if (tid < h) {
-            sred[0][tid] += sred[0][tid + h];
-            sred[1][tid] += sred[1][tid + h];
+#pragma unroll 1
+        for (int q = 0; q < 4; q++) {
+            const int r = wave * 2 + (q >> 1), c = (q & 1) * 16 + j;
+            const floatx4 acc = conv_out_mfma(sx, WS, r, c, lane, A);
+            const int gy = y0 + r, gx = x0 + c;
+            if (lane < 16) {  // every pixel of the tile writes its sums: zeros outside the image
+                float s2 = 0.0f, x2 = 0.0f;
+                if (gy < H && gx < W) {
+                    const float s = acc[0] + b5;
+                    const long p = ibase + (long)gy * W + gx;
+                    const float xn = (PF ? xv[q] : x[p]) / scale;
+                    const float v = xn + s;
+                    if (yn) yn[p] = v;
+                    if (y) y[p] = v * scale;
+                    s2 = s * s;
+                    x2 = xn * xn;
+                }
+                sred[0][r * TW + c] = s2;
+                sred[1][r * TW + c] = x2;
+            }
+        }
+        __syncthreads();  // sx reads and sred writes done
+        if (PF && next < ntiles) put_tile(sx, tid, pv);
+        if (tid < 128) {
+            sred[0][tid] += sred[0][tid + 128];
+            sred[1][tid] += sred[1][tid + 128];
         }
         __syncthreads();
-    }
-    if (tid == 0 && stats) {
-        stats[2 * blockIdx.x] = sred[0][0];
-        stats[2 * blockIdx.x + 1] = sred[1][0];
+        if (tid < 64) {
+            float r0 = sred[0][tid] + sred[0][tid + 64], r1 = sred[1][tid] + sred[1][tid + 64];
+#pragma unroll
+            for (int h = 32; h > 0; h >>= 1) {
+                r0 += __shfl_down(r0, h, 64);
+                r1 += __shfl_down(r1, h, 64);
+            }
+            if (tid == 0 && stats) {
+                stats[2L * tile] = r0;
+                stats[2L * tile + 1] = r1;
+            }
+        }
+        if (PF) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) xv[q] = xn_next[q];
+        }
     }
 }
 
@@ -1170,22 +1291,40 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     const long npx = (long)n * H * W;
     const uint16_t *pk = reinterpret_cast<const uint16_t *>(packed);
     const int nconv = ntiles < MAX_CONV_BLOCKS ? ntiles : MAX_CONV_BLOCKS;
+    // a 32 -> 32 forward layer, with the next-tile prefetch when an image's
+    // activations fit a buffer view (< 2^31 bytes)
+    // (its workgroups: exactly those resident at once, each looping over tiles)
+    const bool pf = (long)H * W * CH * (long)sizeof(uint16_t) < (1L << 31);
+    int npf = 0;
+    if (pf) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_conv32<0, true>, 256, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const long want = (long)per_cu * (gr_cu_count() > 0 ? gr_cu_count() : 256);
+        npf = (int)std::min<long>(ntiles, want);
+    }
+    auto conv32 = [&](const uint16_t *in, const uint16_t *wl, const float *b, uint16_t *out) {
+        if (pf)
+            hipLaunchKernelGGL((k_conv32<0, true>), dim3(npf), dim3(256), 0, s, in, wl, b, (const uint16_t *)nullptr,
+                               out, H, W, ntiles, tm);
+        else
+            hipLaunchKernelGGL((k_conv32<0, false>), dim3(nconv), dim3(256), 0, s, in, wl, b,
+                               (const uint16_t *)nullptr, out, H, W, ntiles, tm);
+    };
     // inference (a3 aliasing a1 and a4 aliasing a2: the caller keeps no
     // activation): the first layer fused into the second
     const bool fused = a3 == a1 && a4 == a2;
     if (fused) {
         hipLaunchKernelGGL(k_conv12, dim3(nconv), dim3(256), 0, s, x, scale, params, pk, a2, H, W, ntiles, tm);
-        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, a2, pk + PACK_LAYER, params + OFF_B3,
-                           (const uint16_t *)nullptr, a3, H, W, ntiles, tm);
-        hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, a3, pk + 2L * PACK_LAYER, params + OFF_B4,
-                           (const uint16_t *)nullptr, a4, H, W, ntiles, tm);
+        conv32(a2, pk + PACK_LAYER, params + OFF_B3, a3);
+        conv32(a3, pk + 2L * PACK_LAYER, params + OFF_B4, a4);
     } else {
         hipLaunchKernelGGL(k_conv_in, dim3(nconv), dim3(256), 0, s, x, scale, params, a1, H, W, ntiles, tm);
         uint16_t *acts[4] = {a1, a2, a3, a4};
         const int boff[3] = {OFF_B2, OFF_B3, OFF_B4};
-        for (int l = 0; l < 3; l++)
-            hipLaunchKernelGGL(k_conv32<0>, dim3(nconv), dim3(256), 0, s, acts[l], pk + (long)l * PACK_LAYER,
-                               params + boff[l], (const uint16_t *)nullptr, acts[l + 1], H, W, ntiles, tm);
+        for (int l = 0; l < 3; l++) conv32(acts[l], pk + (long)l * PACK_LAYER, params + boff[l], acts[l + 1]);
     }
     float *stats = nullptr;
     if (slice_stats) {
@@ -1195,7 +1334,19 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
               al(sizeof(float) * (size_t)MAX_WG_BLOCKS * WG_REC);
         stats = (float *)wb;
     }
-    hipLaunchKernelGGL(k_conv_out, dim3(ntiles), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W, tm);
+    if (pf) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)k_conv_out<true>, 256, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const int nout = (int)std::min<long>(ntiles, (long)per_cu * (gr_cu_count() > 0 ? gr_cu_count() : 256));
+        hipLaunchKernelGGL(k_conv_out<true>, dim3(nout), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W,
+                           ntiles, tm);
+    } else {
+        hipLaunchKernelGGL(k_conv_out<false>, dim3(ntiles), dim3(256), 0, s, a4, x, scale, params, y, yn, stats, H, W,
+                           ntiles, tm);
+    }
     if (slice_stats) {
         const int per = tm.tiles_x * tm.tiles_y;
         hipLaunchKernelGGL(k_sum_groups, dim3(2 * n), dim3(256), 0, s, stats, n, per, 2, 1.0,

@@ -416,7 +416,7 @@ static int device_attr(hipDeviceAttribute_t attr, std::atomic<int> (&cache)[64])
     return v > 0 ? v : 0;
 }
 
-static int gr_cu_count() {
+int gr_cu_count() {
     static std::atomic<int> cache[64];
     return device_attr(hipDeviceAttributeMultiprocessorCount, cache);
 }
