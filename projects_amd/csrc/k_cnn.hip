@@ -62,9 +62,12 @@ constexpr int MAX_CONV_BLOCKS = 2048;  // 8 per CU: each loops over tiles with i
 static_assert(N_PARAMS == 28353, "parameter layout");
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
-    // round to nearest, ties to even (finite inputs)
-    const uint32_t u = __float_as_uint(f);
-    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+    // round to nearest, ties to even: one v_cvt_pk_bf16_f32 (for two values)
+    // instead of four integer operations per value.  On every non-NaN float
+    // it equals the integer rounding (u + 0x7FFF + ((u >> 16) & 1)) >> 16 of
+    // the bit pattern u, the oracle's (checked over all 2^32 patterns on
+    // MI355X: no difference).
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float round_bf16(float f) { return from_bf16(to_bf16(f)); }
