@@ -531,7 +531,7 @@ int gr_set_error(int code, const char *msg) { return fail(code, msg); }
 
 size_t gr_plan_workspace_bytes(const gr_plan *pl, int max_pairs) {
     const GrGeom &g = pl->g;
-    const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
+    const size_t spec_pp = (size_t)pl->prm.n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * gr_z_cols(g.out_n);
     const size_t spec_pairs = (size_t)(max_pairs + kSpecGroup - 1) / kSpecGroup * kSpecGroup;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     return 2 * up(sizeof(float2) * spec_pp * spec_pairs) + up(sizeof(float2) * ht_pp * max_pairs) +
@@ -685,7 +685,7 @@ int gr_plan_create_impl(const gr_params *params, gr_plan **plan_out, bool alloc_
         }
     }
     // workspace
-    const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * g.out_n;
+    const size_t spec_pp = (size_t)n_ang * g.half, ht_pp = (size_t)g.M * g.M, z_pp = (size_t)g.M * gr_z_cols(g.out_n);
     const size_t per_pair = sizeof(float2) * (2 * spec_pp + ht_pp + z_pp);
     int mp = params->max_pairs;
     if (mp <= 0) {

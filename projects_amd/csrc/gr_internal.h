@@ -26,6 +26,14 @@ struct GrGeom {
     int fft_radix = 16, grid_wpb = 2;
 };
 
+// Z, the row-transformed grid between K3 and K4, is held column-blocked:
+// per pair, [window column / 4][row b < M][window column % 4] float2, so the
+// four columns a K4 workgroup transforms are one contiguous M x 32-byte block
+// (the window's width rounded up to a multiple of four; the padding columns
+// are never written nor stored).
+__host__ __device__ __forceinline__ long gr_z_cols(long out_n) { return (out_n + 3) / 4 * 4; }
+__host__ __device__ __forceinline__ long gr_z_index(long M, long b, long jj) { return ((jj >> 2) * M + b) * 4 + (jj & 3); }
+
 // Host side (gr_tables.cpp): sizes and the reference's float tables.
 int gr_geom_from_params(const gr_params *p, GrGeom *g, const char **why);
 int gr_find_pswf(float C, gr_pswf *out);

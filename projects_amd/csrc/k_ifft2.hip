@@ -37,7 +37,8 @@ struct RowsArgs {
 
 // Output hook of the row transform: FFT output i lands in window column
 // jj = j' - out_off, where corner_index(j') = i (outputs that map to no
-// window column are dropped).
+// window column are dropped), at Z's column-blocked place for the row (dst =
+// the pair's Z + 4 b).
 struct WindowRow {
     float2 *dst;
     int M, m02, off, out_n;
@@ -46,9 +47,20 @@ struct WindowRow {
     __device__ __forceinline__ void operator()(int i, float2 v, int) const {
         const int jp = i >= M - m02 ? i - (M - m02) : i + m02;
         const int jj = jp - off;
-        if (jj >= 0 && jj < out_n) dst[jj] = v;
+        if (jj >= 0 && jj < out_n) dst[(long)(jj >> 2) * M * 4 + (jj & 3)] = v;
     }
 };
+
+// Row of Z written by row-transform workgroup x (of nx, CW rows each): the
+// workgroups are dealt to the 8 XCDs round robin, so x, x + 8, ... share one
+// XCD's L2; numbering them so that the 4 rows whose 32-byte pieces make up
+// each 128-byte line of the column-blocked Z come from one XCD, one after
+// another, lets its L2 merge the pieces into whole lines.
+__device__ __forceinline__ int row_group(int x, int nx, int CW) {
+    const int G = 4 / CW;  // workgroups per 4 rows
+    if (G <= 1 || nx % (8 * G)) return x;
+    return (x / (8 * G)) * (8 * G) + (x % 8) * G + (x / 8) % G;
+}
 
 // Loads and LDS stores in loops unrolled to the per-thread bound (M <= 4 MAXB
 // nthr), every load issued before the first store: one memory latency per
@@ -60,7 +72,7 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
     constexpr int EPT = 4 * MAXB;
     const int nthr = (int)blockDim.x / CW;
     const int grp = (int)threadIdx.x / nthr, tid = (int)threadIdx.x % nthr;
-    const int b = blockIdx.x * CW + grp, p = blockIdx.y;
+    const int b = row_group((int)blockIdx.x, (int)gridDim.x, CW) * CW + grp, p = blockIdx.y;
     const int M = A.M;
     float2 *row = lds + (long)grp * M;
     const float2 *src = A.ht + ((long)p * M + b) * M;
@@ -78,7 +90,7 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
         if (i < M) row[gr_lds_idx(i)] = v[r];
     }
     __syncthreads();
-    float2 *dst = A.z + ((long)p * M + b) * A.out_n;
+    float2 *dst = A.z + (long)p * M * gr_z_cols(A.out_n) + 4L * b;
     const int out_n = (int)A.out_n, off = (int)A.out_off, m02 = (int)A.M02;
     // when the last pass is radix 16, the window's outputs go to HBM straight
     // from its registers: output i is window column j' = corner_index^-1(i)
@@ -94,7 +106,7 @@ __global__ void __launch_bounds__(1024) k_ifft_rows(RowsArgs A) {
 #pragma unroll
     for (int r = 0; r < EPT; r++) {
         const int jj = tid + r * nthr;
-        if (jj < out_n) dst[jj] = o[r];
+        if (jj < out_n) dst[(long)(jj >> 2) * M * 4 + (jj & 3)] = o[r];
     }
 }
 
@@ -159,21 +171,21 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
     const int nthr = CW ? (int)blockDim.x / CW : (int)blockDim.x;
     const int tid = CW ? (int)threadIdx.x % nthr : (int)threadIdx.x;
     const int grp = CW ? (int)threadIdx.x / nthr : 0;
-    const float2 *zp = A.z + (long)p * M * A.out_n;
+    const float2 *zp = A.z + (long)p * M * gr_z_cols(A.out_n);  // column-blocked
     const int out_n = (int)A.out_n;
     constexpr int EPT = 4 * MAXB;  // rows per thread and column: M <= 4 MAXB nthr
     if (CW) {
         // unrolled, every load before the first LDS store (buffer loads: a
         // column past the window reads zeros without a branch); the CW
         // columns of a row come from neighbouring lanes
-        const GrBuf zb = gr_buf(zp, (unsigned)((long)M * out_n * sizeof(float2)));
+        const GrBuf zb = gr_buf(zp, (unsigned)((long)M * gr_z_cols(out_n) * sizeof(float2)));
         float2 v[EPT];
 #pragma unroll
         for (int r = 0; r < EPT; r++) {
             const int idx = (int)threadIdx.x + r * (int)blockDim.x;
             const int b = idx / CW, c = idx - b * CW;
             const bool in = b < M && c0 + c < out_n;
-            v[r] = gr_buf_ld2(zb, in ? (unsigned)(((long)b * out_n + c0 + c) * sizeof(float2)) : kGrBufNone);
+            v[r] = gr_buf_ld2(zb, in ? (unsigned)(gr_z_index(M, b, c0 + c) * sizeof(float2)) : kGrBufNone);
         }
 #pragma unroll
         for (int r = 0; r < EPT; r++) {
@@ -182,13 +194,13 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
             if (b < M) cols[c * gr_lds_len(M) + gr_lds_idx(b)] = v[r];
         }
     } else if (cw == 1) {
-        const float2 *zc = zp + c0;
-        for (int b = tid; b < M; b += nthr) cols[gr_lds_idx(b)] = c0 < out_n ? zc[(long)b * out_n] : make_float2(0.0f, 0.0f);
+        for (int b = tid; b < M; b += nthr)
+            cols[gr_lds_idx(b)] = c0 < out_n ? zp[gr_z_index(M, b, c0)] : make_float2(0.0f, 0.0f);
     } else {
         for (int idx = threadIdx.x; idx < cw * M; idx += blockDim.x) {  // CW: cw * M <= 20480; else <= 16384
             const int b = idx / cw, c = idx - b * cw;
             cols[c * gr_lds_len(M) + gr_lds_idx(b)] =
-                (c0 + c < out_n) ? zp[(long)b * out_n + c0 + c] : make_float2(0.0f, 0.0f);
+                (c0 + c < out_n) ? zp[gr_z_index(M, b, c0 + c)] : make_float2(0.0f, 0.0f);
         }
     }
     __syncthreads();
@@ -283,12 +295,13 @@ __global__ void __launch_bounds__(kPfThreads) k_ifft_cols_pf(ColsArgs A, int n_i
         const int t = (int)threadIdx.x, lc = t % kPfCw;
         const int p = item / ngroups, x = item - p * ngroups;
         const int c0 = (int)column_group((unsigned)x, (unsigned)ngroups) * kPfCw;
-        const GrBuf zb = gr_buf(A.z + (long)p * kPfM * out_n, (unsigned)((long)kPfM * out_n * sizeof(float2)));
+        // the group's column block: 4096 rows x 32 bytes, contiguous
+        const GrBuf zb = gr_buf(A.z + (long)p * kPfM * gr_z_cols(out_n) + (long)c0 * kPfM,
+                                (unsigned)(kPfM * kPfCw * sizeof(float2)));
         const bool in = c0 + lc < out_n;
-        const unsigned o0 = in ? (unsigned)(((t / kPfCw) * out_n + c0 + lc) * sizeof(float2)) : kGrBufNone;
+        const unsigned o0 = in ? (unsigned)(((t / kPfCw) * kPfCw + lc) * sizeof(float2)) : kGrBufNone;
 #pragma unroll
-        for (int r = 0; r < 16; r++)
-            v[r] = gr_buf_ld2(zb, o0, (unsigned)(r * kPfNthr * out_n * sizeof(float2)));
+        for (int r = 0; r < 16; r++) v[r] = gr_buf_ld2(zb, o0, (unsigned)(r * kPfNthr * kPfCw * sizeof(float2)));
         const int grp = t / kPfNthr;
         cu_next = A.winv[off + (c0 + grp < out_n ? c0 + grp : out_n - 1)];
     };
@@ -338,6 +351,75 @@ __global__ void __launch_bounds__(kPfThreads) k_ifft_cols_pf(ColsArgs A, int n_i
     }
 }
 
+// Persistent row transform for the same geometry (M = 4096, radix 16, window
+// <= 2048): an item is four neighbouring rows b0 .. b0+3 of Ht (one 8-row tile
+// row: one active-tile interval), one per 256-thread group; each thread loads
+// elements j + 256 r of its row (buffer loads: inactive tiles read zeros, no
+// branch), runs the first radix-16 butterfly j from those registers, then
+// issues the next item's loads, which stay in flight through passes 2 and 3
+// (twiddles from LDS).  The epilogue writes Z column-blocked in whole lines:
+// sixteen consecutive lanes store one 128-byte line -- the four rows' four
+// columns of one column block.  Same arithmetic and output values as
+// k_ifft_rows: bitwise.
+__global__ void __launch_bounds__(kPfThreads) k_ifft_rows_pf(RowsArgs A, int n_items) {
+    extern __shared__ float2 lds[];
+    float2 *tw = lds + kPfCw * kPfColStride;
+    const int out_n = (int)A.out_n, off = (int)A.out_off, m02 = (int)A.M02;
+    const int groups = kPfM / kPfCw;  // items per pair
+    for (int i = (int)threadIdx.x; i < kPfTw; i += kPfThreads) tw[i] = A.tw[i];
+    float2 v[16];
+    // the active-tile interval of an item's tile row (its four rows share
+    // one), fetched an item ahead: waited for only where the loads issued
+    // before it are waited for anyway
+    auto interval = [&](int item) {
+        const int p = item / groups;
+        return item < n_items ? A.row_tiles[((item - p * groups) * kPfCw) >> 3] : make_int2(1, 0);
+    };
+    auto load = [&](int item, int2 lh, int t) {  // t: threadIdx.x (an opaque copy inside the loop)
+        const int grp = t / kPfNthr, tid = t % kPfNthr;
+        const int p = item / groups, b0 = (item - p * groups) * kPfCw;
+        const GrBuf hb = gr_buf(A.ht + ((long)p * kPfM + b0) * kPfM, (unsigned)(kPfCw * kPfM * sizeof(float2)));
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int i = tid + r * kPfNthr, tx = i >> 3;
+            const bool in = tx >= lh.x && tx <= lh.y;
+            v[r] = gr_buf_ld2(hb, in ? (unsigned)((grp * kPfM + i) * sizeof(float2)) : kGrBufNone);
+        }
+    };
+    int item = (int)blockIdx.x;
+    if (item < n_items) load(item, interval(item), (int)threadIdx.x);
+    int2 lh_next = interval(item + (int)gridDim.x);
+    __syncthreads();  // twiddles in place
+    while (item < n_items) {  // the same trip count in every thread of the block
+        int t = (int)threadIdx.x;
+        asm volatile("" : "+v"(t));  // indices recomputed per item (as k_ifft_cols_pf)
+        const int grp = t / kPfNthr, tid = t % kPfNthr;
+        float2 *row = lds + grp * kPfColStride;
+        gr_fft_first_pass16<-1>(row, v, tid);
+        const int cur = item;
+        item += (int)gridDim.x;
+        __syncthreads();
+        if (item < n_items) {
+            load(item, lh_next, t);
+            lh_next = interval(item + (int)gridDim.x);
+        }
+        gr_lds_fft<-1, 4, GrFftToLds, true>(row, kPfM, tw, tid, kPfNthr, 16, GrFftToLds(), true);
+        const int p = cur / groups, b0 = (cur - p * groups) * kPfCw;
+        float2 *zp = A.z + (long)p * kPfM * gr_z_cols(out_n);
+#pragma unroll
+        for (int r = 0; r < 8; r++) {  // 4 rows x up to 2048 columns = 8192 outputs
+            const int L = t + r * kPfThreads, blk = L >> 4, rr = (L >> 2) & 3, c = L & 3;
+            const int jj = 4 * blk + c;
+            if (jj < out_n) {
+                const int k = off + jj;
+                zp[((long)blk * kPfM + b0 + rr) * 4 + c] =
+                    lds[rr * kPfColStride + gr_lds_idx(k < m02 ? kPfM - m02 + k : k - m02)];
+            }
+        }
+        __syncthreads();  // the rows are free for the next item's first pass
+    }
+}
+
 inline int maxb_of(long N, int nthr) { return (int)((N / 4 + nthr - 1) / nthr); }
 
 __global__ void k_row_tiles(int2 *out, int ntr, float M2, float rmax) {
@@ -349,6 +431,32 @@ __global__ void k_row_tiles(int2 *out, int ntr, float M2, float rmax) {
 }
 
 }  // namespace
+
+// A figure of the current device, queried once per device (0 when the query
+// fails).
+static int device_attr(hipDeviceAttribute_t attr, std::atomic<int> (&cache)[64]) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v == 0) {
+        if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = -1;
+        cache[dev].store(v, std::memory_order_relaxed);
+    }
+    return v > 0 ? v : 0;
+}
+
+int gr_cu_count() {
+    static std::atomic<int> cache[64];
+    return device_attr(hipDeviceAttributeMultiprocessorCount, cache);
+}
+
+// The persistent column transform needs the whole 160 KiB of a CU's LDS.
+static bool pf_fits() {
+    static std::atomic<int> optin[64], plain[64];
+    const int lds = std::max(device_attr(hipDeviceAttributeSharedMemPerBlockOptin, optin),
+                             device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, plain));
+    return gr_cu_count() > 0 && (size_t)lds >= kPfLds;
+}
 
 hipError_t gr_launch_row_tiles(const GrGeom &g, int2 *out, hipStream_t s) {
     const int ntr = (int)(g.M >> 3);
@@ -365,6 +473,14 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
                                int pairs, hipStream_t s) {
     RowsArgs A{ht, z, t.tw_grid, t.row_tiles, (int)g.M, g.fft_radix, g.M02, g.out_off, g.out_n};
     const int nthr = gr_fft_threads(g.M);
+    if (g.M == kPfM && nthr == kPfNthr && g.fft_radix == 16 && g.out_n <= 8 * kPfNthr && pf_fits()) {
+        const int n_items = (int)(g.M / kPfCw) * pairs;
+        const int nblk = std::min(n_items, gr_cu_count());
+        hipError_t e = gr_allow_lds(k_ifft_rows_pf, kPfLds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_ifft_rows_pf, dim3(nblk), dim3(kPfThreads), kPfLds, s, A, n_items);
+        return hipGetLastError();
+    }
     int cw = kRowsPerBlock;  // rows side by side: a power of two dividing M, within 1024 threads and the LDS
     while (cw > 1 && ((long)cw * nthr > 1024 || (long)cw * g.M * 8 > 163840 || g.M % cw)) cw /= 2;
     const dim3 grid((unsigned)(g.M / cw), pairs), block(cw * nthr);
@@ -402,32 +518,6 @@ hipError_t gr_launch_ifft_rows(const GrGeom &g, const GrDevTables &t, const floa
 // bound the kernel and a line then serves that many columns (at M = 4096 on
 // MI355X: 0.57 ms per launch of 8 pairs with 1, 0.50 with 2, 0.48 with 4).
 // Callers may ask for a width (gr_plan.cols_per_block).
-
-// A figure of the current device, queried once per device (0 when the query
-// fails).
-static int device_attr(hipDeviceAttribute_t attr, std::atomic<int> (&cache)[64]) {
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
-    int v = cache[dev].load(std::memory_order_relaxed);
-    if (v == 0) {
-        if (hipDeviceGetAttribute(&v, attr, dev) != hipSuccess || v <= 0) v = -1;
-        cache[dev].store(v, std::memory_order_relaxed);
-    }
-    return v > 0 ? v : 0;
-}
-
-int gr_cu_count() {
-    static std::atomic<int> cache[64];
-    return device_attr(hipDeviceAttributeMultiprocessorCount, cache);
-}
-
-// The persistent column transform needs the whole 160 KiB of a CU's LDS.
-static bool pf_fits() {
-    static std::atomic<int> optin[64], plain[64];
-    const int lds = std::max(device_attr(hipDeviceAttributeSharedMemPerBlockOptin, optin),
-                             device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, plain));
-    return gr_cu_count() > 0 && (size_t)lds >= kPfLds;
-}
 
 static bool cols_side_by_side(long M, int cw) {
     return cw >= 1 && cw <= 8 && (cw & (cw - 1)) == 0 && (long)cw * gr_fft_threads(M) <= 1024 &&

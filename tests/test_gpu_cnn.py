@@ -55,6 +55,45 @@ def make(seed, n=2, H=19, W=45, scale=1.7):
     return net, {k: v.astype(np.float32) for k, v in pd.items()}, x, t
 
 
+def test_weight_packing_rounds_like_the_oracle():
+    """The kernels' BF16 rounding (the hardware convert) against the oracle's
+    integer round-to-nearest-even, bitwise, through the weight packing of the
+    three 32 -> 32 layers: exact ties to even and to odd, one unit either side
+    of a tie, subnormals (ties included), the largest finites (rounding to
+    infinity), signed zeros and infinities, and random floats of every
+    exponent."""
+    rng = np.random.default_rng(77)
+    n = 3 * 9 * K.C * K.C
+    hi = rng.integers(0, 1 << 16, n, dtype=np.uint64)
+    lo = rng.choice(np.array([0x8000, 0x7FFF, 0x8001, 0x0000, 0xFFFF], np.uint64), n)
+    u = (hi << 16) | lo
+    special = np.array([0x00008000, 0x00018000, 0x00007FFF, 0x00000001, 0x807F8000, 0x7F7FFFFF, 0x7F7F8000,
+                        0x7F7E8000, 0xFF7FFFFF, 0x00000000, 0x80000000, 0x7F800000, 0xFF800000, 0x3F808000,
+                        0x3F818000], np.uint64)
+    u[:special.size] = special
+    u[-4096:] = rng.integers(0, 1 << 32, 4096, dtype=np.uint64)
+    w = u.astype(np.uint32).view(np.float32)
+    w = np.where(np.isnan(w), np.float32(1.0), w).astype(np.float32)
+    net, pd, _, _ = make(5)
+    sizes = {name: int(np.prod(shape)) for name, shape in Lrn.PARAM_SHAPES}
+    off, flat = 0, net.params.cpu().numpy().copy()
+    per = 9 * K.C * K.C
+    for name, _ in Lrn.PARAM_SHAPES:
+        if name in ("W2", "W3", "W4"):
+            l = int(name[1]) - 2
+            flat[off:off + per] = w[l * per:(l + 1) * per]
+        off += sizes[name]
+    net.params.copy_(torch.from_numpy(flat))
+    net.pack()
+    torch.cuda.synchronize()
+    packed = net.packed.cpu().numpy().view(np.uint16)
+    for l in range(3):
+        W = w[l * per:(l + 1) * per].reshape(K.C, K.C, 9)  # [co][ci][tap]
+        want = (K.bf16(W).view(np.uint32) >> 16).astype(np.uint16)
+        fwd = packed[l * 2 * per:l * 2 * per + per].reshape(9, K.C, K.C)  # [tap][co][ci]
+        assert np.array_equal(fwd, want.transpose(2, 0, 1)), l
+
+
 def test_forward_layers_match_oracle():
     net, pd, x, _ = make(1)
     xg = torch.from_numpy(x).cuda()
