@@ -667,10 +667,10 @@ def main():
     # measured HBM bytes per launch of this kernel at this workload, from the
     # FETCH_SIZE / WRITE_SIZE counter runs kept beside the profile summaries
     tf = os.path.join(ROOT, "profiles", "gridrec_plain_1500x2048_hbm_traffic.json")
-    if os.path.exists(tf) and (nang, ndet, inf["max_pairs"], a.accum) == (1500, 2048, 8, "compensated"):
+    if os.path.exists(tf) and (nang, ndet, a.accum) == (1500, 2048, "compensated"):
         with open(tf) as f:
             k = json.load(f)["kernels"].get(dom)
-        if k and k["pairs_per_launch"] == 8 and k.get("grid_op", "off") == grid_op_used:
+        if k and k["pairs_per_launch"] == inf["max_pairs"] and k.get("grid_op", "off") == grid_op_used:
             roofline["traffic"] = k["total_bytes"]
             roofline["traffic_read_write"] = [k["read_bytes"], k["write_bytes"]]
             roofline["traffic_source"] = "profiles/gridrec_plain_1500x2048_hbm_traffic.md"

@@ -552,7 +552,12 @@ struct GopArgs {
     const uint4 *sop_tile;
     const uint2 *sop_ch;
     const unsigned *sop_keys, *sop_ent;
-    int sop_waves;  // persistent light waves per sector
+    int sop_waves;  // persistent light waves per sector (and group of pairs)
+    // groups of P pairs whose workgroups are interleaved so that the groups'
+    // waves walk the same tiles side by side on one XCD and read each entry
+    // through its L2 once (1: a launch of up to P pairs; entries streamed past
+    // the caches)
+    int sop_groups;
     // the heavy work items (tile, segment), heavy tiles cut into segments
     // (GrDevTables), and their partials
     const uint2 *hitems;
@@ -565,8 +570,11 @@ struct GopArgs {
     int part_pairs;
 };
 
-__device__ __forceinline__ uint2 gop_load(const uint2 *p) {
-    const unsigned long long u = __builtin_nontemporal_load(reinterpret_cast<const unsigned long long *>(p));
+// An entry of the streamed operator: past the caches when one group of pairs
+// reads it (no other reader), cached when the interleaved groups share it.
+__device__ __forceinline__ uint2 gop_load(const uint2 *p, bool shared) {
+    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
+    const unsigned long long u = shared ? *q : __builtin_nontemporal_load(q);
     return make_uint2((unsigned)u, (unsigned)(u >> 32));
 }
 
@@ -621,7 +629,7 @@ __device__ __forceinline__ void grid_op_heavy(const GridArgs &A, const GopArgs &
         return;
     }
     auto entry = [&](unsigned k) {
-        const uint2 t = gop_load(e + 64ul * min(k, n - 1));
+        const uint2 t = gop_load(e + 64ul * min(k, n - 1), O.sop_groups > 1);
         return k < n ? t : make_uint2(0u, 0u);
     };
     // software-pipelined: the next batch's entries are in flight while this
@@ -766,7 +774,10 @@ __device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs 
         for (unsigned c = 0; c < nch; c++) {
             const unsigned rows = h.x & ~kSopMerge, ns = h.y;
             const unsigned last = rows ? rows - 1u : 0u;
-            auto eload = [&](unsigned k) { return __builtin_nontemporal_load(ep + 64ul * min(k, last)); };
+            const bool shared = O.sop_groups > 1;
+            auto eload = [&](unsigned k) {
+                return shared ? ep[64ul * min(k, last)] : __builtin_nontemporal_load(ep + 64ul * min(k, last));
+            };
             sop_stage<P>(k0, base1, base2, stg, 0);
             if (ns > 64u) sop_stage<P>(k1, base1, base2, stg, 64);
             // the next chunk (or this one again past the last)
@@ -846,7 +857,7 @@ __device__ __forceinline__ void grid_op_tile(const GridArgs &A, const GopArgs &O
     if (n) {
         const uint2 zero = make_uint2(0u, 0u);
         auto row = [&](unsigned k) {
-            const uint2 t = gop_load(e + 64ul * min(r0 + k, rows - 1u));
+            const uint2 t = gop_load(e + 64ul * min(r0 + k, rows - 1u), O.sop_groups > 1);
             return k < len ? t : zero;
         };
         auto consume = [&](uint2 t, const float2 (&v)[P]) {
@@ -940,14 +951,25 @@ __global__ void __launch_bounds__(256, 4) k_grid_op(GridArgs A, GopArgs O) {
         for (int i = threadIdx.x; i <= A.g.ltbl; i += blockDim.x) smem[i] = A.wtbl[i];
         __syncthreads();
     }
-    const int pair0 = A.pair_base + blockIdx.y * P;
+    // SOP with G = sop_groups > 1 groups of pairs in one launch: workgroup x
+    // is workgroup bx of group gy, numbered so that the G groups' workgroups
+    // of one slot are x, x + 8, ..., x + 8 (G - 1): dealt to the same XCD one
+    // after another (light ones first, then the heavy ones, interleaved alike)
+    int bx = (int)blockIdx.x, gy = (int)blockIdx.y;
+    if (SOP && O.sop_groups > 1) {
+        const int G = O.sop_groups, nl = 2 * O.sop_waves;  // nl and the heavy count: multiples of 8
+        const int x = bx < nl * G ? bx : bx - nl * G;
+        gy = (x >> 3) % G;
+        bx = (x / (8 * G)) * 8 + (x & 7) + (bx < nl * G ? 0 : nl);
+    }
+    const int pair0 = A.pair_base + gy * P;
     const int nvalid = min(P, A.pair_end - pair0);
     // SOP: the persistent light workgroups first (they start at once and
     // hold most of the machine), the heavy items after them in the slots
     // left; otherwise the heavy items first (their long chains start early)
     const int hb = heavy_blocks(O.n_hitems);
     const int nlight = SOP ? 2 * O.sop_waves : 0;  // kGridSectors * sop_waves / 4
-    const int hbx = (int)blockIdx.x - nlight;
+    const int hbx = bx - nlight;
     if (hbx >= 0 && hbx < hb) {
         // wave: work item hbx * 4 + wave -- a cut tile's segment (the P pairs
         // into partials), or a heavy tile one pair per wave
@@ -967,7 +989,7 @@ __global__ void __launch_bounds__(256, 4) k_grid_op(GridArgs A, GopArgs O) {
         // persistent: workgroup b (XCD b mod 8) serves sector b mod 8 with
         // sop_waves waves, each taking every sop_waves-th light slot
         const int wv = __builtin_amdgcn_readfirstlane(wave);
-        const int lb = blockIdx.x;
+        const int lb = bx;
         char *stg = reinterpret_cast<char *>(smem + A.lds_wtbl) + (size_t)wv * kSopSlots * P * sizeof(float2);
         grid_sop_light<ACCUM, P>(A, O, lb % kGridSectors, O.n_heavy + (lb / kGridSectors) * 4 + wv, O.sop_waves,
                                  pair0, nvalid, lane, stg, smem);
@@ -995,16 +1017,22 @@ hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_ba
             return hipGetLastError();
         }
         // the staged light tiles: persistent workgroups, kSopBlocksPerSector
-        // per sector and group, fewer when the sector has fewer slots
-        const int lb = std::min(light, kSopBlocksPerSector);
+        // per sector shared by the launch's groups of pairs (up to 8 groups
+        // interleaved), fewer when the sector has fewer slots
+        const int G = groups <= 8 ? groups : 1;
+        const int lb = std::max(1, std::min(light, kSopBlocksPerSector / G));
         const int sblocks = heavy_blocks(O->n_hitems) + kGridSectors * lb;
         GopArgs o = *O;
         o.sop_waves = 4 * lb;
+        o.sop_groups = G;
 
         const size_t lds = lds_base + sizeof(float2) * 4 * kSopSlots * P;
         hipError_t e = gr_allow_lds(k_grid_op<ACCUM, P, true>, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_grid_op<ACCUM, P, true>), dim3(sblocks, groups), dim3(256), lds, s, A, o);
+        if (G > 1)
+            hipLaunchKernelGGL((k_grid_op<ACCUM, P, true>), dim3(G * sblocks, 1), dim3(256), lds, s, A, o);
+        else
+            hipLaunchKernelGGL((k_grid_op<ACCUM, P, true>), dim3(sblocks, groups), dim3(256), lds, s, A, o);
         return hipGetLastError();
     }
     const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
@@ -1089,6 +1117,7 @@ hipError_t gr_launch_grid(const GrGeom &g, const GrDevTables &t, int n_ang, long
     // segments only with compensated sums (their merges are the cut points)
     const bool cut = t.hmeta && t.n_split > 0 && accum == GR_ACCUM_COMPENSATED;
     GopArgs O{};
+    O.sop_groups = 1;
     O.gop = t.gop;
     O.tile_n = t.gop_n;
     O.tile_off = t.gop_off;
