@@ -842,7 +842,10 @@ int exec_pairs(gr_plan *pl, const float *const *G1, const float *const *G2, floa
     const long on = g.out_n;
     const size_t in_pp = (size_t)2 * n_ang * nd, out_pp = (size_t)2 * on * on;  // floats per pair
     if (!pl->stage_pairs) {
-        const int sp = pl->max_pairs;
+        // chunks of at most 8 pairs: the host's packing and PCIe, not the
+        // kernels, bound this path, and more chunks overlap more of them
+        // (the pipeline's fill and drain are one chunk each)
+        const int sp = std::min(pl->max_pairs, 8);
         auto setup = [&]() -> int {
             for (int b = 0; b < 2; b++) {
                 GR_HIP(hipHostMalloc((void **)&pl->h_in[b], sizeof(float) * in_pp * sp));
