@@ -91,9 +91,10 @@ struct GrDevTables {
     const int2 *row_tiles = nullptr;
 };
 
-// Compute units of the current device (queried once per device; 0 when the
-// query fails).
+// Compute units of the current device, and the most LDS one workgroup may
+// take (queried once per device; 0 when the query fails).
 int gr_cu_count();
+int gr_max_lds_per_block();
 
 // ---- kernel launchers (one per .hip file) ----------------------------------
 

@@ -450,13 +450,14 @@ int gr_cu_count() {
     return device_attr(hipDeviceAttributeMultiprocessorCount, cache);
 }
 
-// The persistent column transform needs the whole 160 KiB of a CU's LDS.
-static bool pf_fits() {
+int gr_max_lds_per_block() {
     static std::atomic<int> optin[64], plain[64];
-    const int lds = std::max(device_attr(hipDeviceAttributeSharedMemPerBlockOptin, optin),
-                             device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, plain));
-    return gr_cu_count() > 0 && (size_t)lds >= kPfLds;
+    return std::max(device_attr(hipDeviceAttributeSharedMemPerBlockOptin, optin),
+                    device_attr(hipDeviceAttributeMaxSharedMemoryPerBlock, plain));
 }
+
+// The persistent transforms need the whole 160 KiB of a CU's LDS.
+static bool pf_fits() { return gr_cu_count() > 0 && (size_t)gr_max_lds_per_block() >= kPfLds; }
 
 hipError_t gr_launch_row_tiles(const GrGeom &g, int2 *out, hipStream_t s) {
     const int ntr = (int)(g.M >> 3);

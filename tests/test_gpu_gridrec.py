@@ -210,6 +210,30 @@ def test_pair_grouping_is_bitwise_neutral(accum, grid_op):
         assert torch.equal(out, ref), mp
 
 
+@pytest.mark.parametrize("grid_op", ["off", "on"])
+def test_pair_grouping_at_4096_is_bitwise_neutral(grid_op):
+    """At the 2048-detector geometry (pdim = M = 4096: the persistent row and
+    column transforms over items of 4 rows / 4 columns of every pair of the
+    launch) and, with the operator, gridding in interleaved groups of 8 pairs
+    (16 pairs: two groups, 3 pairs: one partial group), every grouping gives
+    bitwise the images of one pair per launch."""
+    rng = np.random.default_rng(4096)
+    n_ang, n_det, ns = 40, 2048, 33  # 17 pairs, the last one odd
+    sino = torch.from_numpy(rng.standard_normal((ns, n_ang, n_det)).astype(np.float32)).cuda()
+    kw = dict(n_ang=n_ang, n_det=n_det, center=1023.6, pad="half", crop=(1024, n_det), grid_op=grid_op)
+    p = gr.Plan(max_pairs=1, **kw)
+    ref = p.recon(sino)
+    torch.cuda.synchronize()
+    p.close()
+    assert torch.isfinite(ref).all() and ref.abs().max() > 0
+    for mp in (3, 4, 5, 8, 16):
+        p = gr.Plan(max_pairs=mp, **kw)
+        out = p.recon(sino)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), mp
+        p.close()
+
+
 def test_column_widths_are_bitwise_neutral():
     """The column transform's workgroup width (columns side by side, or one
     after another) changes only the launch shape: every width a plan accepts
