@@ -210,6 +210,23 @@ def test_pair_grouping_is_bitwise_neutral(accum, grid_op):
         assert torch.equal(out, ref), mp
 
 
+def test_many_interleaved_groups_are_bitwise_neutral():
+    """Launches of 24 and 64 pairs grid in 3 and 8 interleaved groups of 8
+    pairs (the staged operator's workgroups of a slot side by side on one
+    XCD), with a one-pair remainder launch; bitwise the images of one pair
+    per launch."""
+    g = O.Geometry(n_ang=72, n_det=96, center=47.25)
+    rng = np.random.default_rng(65)
+    ns = 130  # 65 pairs
+    sino = torch.from_numpy(rng.standard_normal((ns, 72, 96)).astype(np.float32)).cuda()
+    ref = plan_for(g, max_pairs=1, grid_op="on").recon(sino)
+    torch.cuda.synchronize()
+    for mp in (24, 64):
+        out = plan_for(g, max_pairs=mp, grid_op="on").recon(sino)
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), mp
+
+
 @pytest.mark.parametrize("grid_op", ["off", "on"])
 def test_pair_grouping_at_4096_is_bitwise_neutral(grid_op):
     """At the 2048-detector geometry (pdim = M = 4096: the persistent row and
@@ -226,7 +243,7 @@ def test_pair_grouping_at_4096_is_bitwise_neutral(grid_op):
     torch.cuda.synchronize()
     p.close()
     assert torch.isfinite(ref).all() and ref.abs().max() > 0
-    for mp in (3, 4, 5, 8, 16):
+    for mp in (3, 4, 5, 8, 16, 40):
         p = gr.Plan(max_pairs=mp, **kw)
         out = p.recon(sino)
         torch.cuda.synchronize()
