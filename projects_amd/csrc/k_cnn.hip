@@ -303,9 +303,9 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
 // ---------------------------------------------------------------------------
 // Inference with the first layer fused into the second (a1 never reaches
 // HBM): the 1 -> 32 layer runs on the tile grown by one pixel -- RR x RC =
-// 10 x 34 pixels, held as 10 rows of three 16-pixel groups (the last partly
-// idle) -- into LDS, zeros outside the image (the padding the second layer's
-// staging gives), then the second layer on the 8 x 32 tile from there.
+// 10 x 34 pixels in 22 groups of 16 -- into LDS, zeros outside the image
+// (the padding the second layer's staging gives), then the second layer on
+// the 8 x 32 tile from there.
 // Every value is computed by the same MFMA sequence and rounding as in the
 // layer-by-layer kernels: bitwise.  (The last layer fused into the fourth the
 // same way was slower: recomputing the 32 -> 32 layer on the grown tile costs
@@ -376,15 +376,19 @@ __global__ void __launch_bounds__(256) k_conv12(const float *__restrict__ x, flo
             }
         }
         __syncthreads();
-        // a1 on the grown tile: 30 groups of 16 pixels over the 4 waves
+        // a1 on the grown tile: its 340 pixels packed row after row into 22
+        // groups of 16 over the 4 waves (a pixel's MFMA column depends on its
+        // own inputs only, so the packing changes no value)
+        constexpr int NG1 = (RR * RC + 15) / 16;
 #pragma unroll 1
-        for (int q = wave; q < RR * 3; q += 4) {
-            const int r = q / 3, c = (q % 3) * 16 + j, cc = c < RC ? c : RC - 1;
-            const bf16x8 B = conv_in_b(sxs, XC2, r, cc, lane);
+        for (int q = wave; q < NG1; q += 4) {
+            const int pix = q * 16 + j, pp = pix < RR * RC ? pix : RR * RC - 1;
+            const int r = pp / RC, c = pp - r * RC;
+            const bf16x8 B = conv_in_b(sxs, XC2, r, c, lane);
             const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
             const floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[0], B, z, 0, 0, 0);
             const floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[1], B, z, 0, 0, 0);
-            if (c < RC) {
+            if (pix < RR * RC) {
                 const int gy = y0 + r - 1, gx = x0 + c - 1;
                 const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
                 put_grown(sx, r, c, 4 * g, acc0, sw + 9 * CH, inside);
