@@ -162,11 +162,13 @@ __device__ __forceinline__ floatx4 conv_out_mfma(const uint16_t *in, int pitch, 
 
 struct TileMap {
     int tiles_x, tiles_y;
-    __device__ void of(int tile, int &img, int &y0, int &x0) const {
+    __device__ void of(int tile, int &img, int &y0, int &x0) const { of_rows(tile, TH, img, y0, x0); }
+    // tiles of th rows (tiles_y counts them)
+    __device__ void of_rows(int tile, int th, int &img, int &y0, int &x0) const {
         const int per = tiles_x * tiles_y;
         img = tile / per;
         const int r = tile - img * per;
-        y0 = (r / tiles_x) * TH;
+        y0 = (r / tiles_x) * th;
         x0 = (r % tiles_x) * TW;
     }
 };
@@ -302,16 +304,19 @@ __global__ void __launch_bounds__(256) k_conv32(const uint16_t *__restrict__ x, 
 
 // ---------------------------------------------------------------------------
 // Inference with the first layer fused into the second (a1 never reaches
-// HBM): the 1 -> 32 layer runs on the tile grown by one pixel -- RR x RC =
-// 10 x 34 pixels in 22 groups of 16 -- into LDS, zeros outside the image
-// (the padding the second layer's staging gives), then the second layer on
-// the 8 x 32 tile from there.
+// HBM), on 16 x 32 output tiles: the 1 -> 32 layer runs on the tile grown
+// by one pixel -- R12 x RC = 18 x 34 pixels in 39 groups of 16 (1.2x the
+// output pixels; 8 x 32 tiles grew 1.33x and had twice the barriers per
+// pixel) -- into LDS, zeros outside the image (the padding the second
+// layer's staging gives), then the second layer on the 16 x 32 tile from
+// there.
 // Every value is computed by the same MFMA sequence and rounding as in the
 // layer-by-layer kernels: bitwise.  (The last layer fused into the fourth the
 // same way was slower: recomputing the 32 -> 32 layer on the grown tile costs
 // 1.9x its MFMA work and the 62 KB workgroup halved the occupancy.)
-constexpr int RR = TH + 2, RC = TW + 2;      // the grown tile (= HS x WS)
-constexpr int XR2 = TH + 4, XC2 = TW + 4;    // its input (x) region, two-pixel halo
+constexpr int TH12 = 16;                     // output tile rows of the fused layers 1 + 2
+constexpr int R12 = TH12 + 2, RC = TW + 2;   // the grown tile
+constexpr int XR12 = TH12 + 4, XC2 = TW + 4; // its input (x) region, two-pixel halo
 
 // Store lane's four channels co0..co0+3 of grown-tile pixel (r, c) as bf16 of
 // relu(acc + bias) -- or zeros for a pixel outside the image.
@@ -335,8 +340,8 @@ __device__ __forceinline__ void put_grown(uint16_t *dst, int r, int c, int co0, 
 __global__ void __launch_bounds__(256) k_conv12(const float *__restrict__ x, float scale,
                                                 const float *__restrict__ prm, const uint16_t *__restrict__ wp,
                                                 uint16_t *__restrict__ y, int H, int W, int ntiles, TileMap tm) {
-    __shared__ float sxs[XR2 * XC2];
-    __shared__ __align__(16) uint16_t sx[RR * RC * PSTR];
+    __shared__ float sxs[XR12 * XC2];
+    __shared__ __align__(16) uint16_t sx[R12 * RC * PSTR];
     __shared__ float sw[9 * CH + 2 * CH];  // W1 (bf16 values), b1, b2
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < 9 * CH; i += 256) sw[i] = round_bf16(prm[OFF_W1 + i]);
@@ -356,39 +361,39 @@ __global__ void __launch_bounds__(256) k_conv12(const float *__restrict__ x, flo
     const int j = lane & 15, g = lane >> 4;
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         int img, y0, x0;
-        tm.of(tile, img, y0, x0);
+        tm.of_rows(tile, TH12, img, y0, x0);
         const long ibase = (long)img * H * W;
         __syncthreads();  // the previous tile's reads of sxs and sx are done
         {
-            constexpr int PER = (XR2 * XC2 + 255) / 256;
+            constexpr int PER = (XR12 * XC2 + 255) / 256;
             float v[PER];
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 const int i = tid + k * 256;
                 const int r = i / XC2, c = i - r * XC2, gy = y0 + r - 2, gx = x0 + c - 2;
-                v[k] = i < XR2 * XC2 && gy >= 0 && gy < H && gx >= 0 && gx < W ? x[ibase + (long)gy * W + gx] : 0.0f;
+                v[k] = i < XR12 * XC2 && gy >= 0 && gy < H && gx >= 0 && gx < W ? x[ibase + (long)gy * W + gx] : 0.0f;
             }
 #pragma unroll
             for (int k = 0; k < PER; k++) {
                 const int i = tid + k * 256;
                 const int r = i / XC2, c = i - r * XC2, gy = y0 + r - 2, gx = x0 + c - 2;
-                if (i < XR2 * XC2) sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(v[k] / scale) : 0.0f;
+                if (i < XR12 * XC2) sxs[i] = (gy >= 0 && gy < H && gx >= 0 && gx < W) ? round_bf16(v[k] / scale) : 0.0f;
             }
         }
         __syncthreads();
-        // a1 on the grown tile: its 340 pixels packed row after row into 22
+        // a1 on the grown tile: its 612 pixels packed row after row into 39
         // groups of 16 over the 4 waves (a pixel's MFMA column depends on its
         // own inputs only, so the packing changes no value)
-        constexpr int NG1 = (RR * RC + 15) / 16;
+        constexpr int NG1 = (R12 * RC + 15) / 16;
 #pragma unroll 1
         for (int q = wave; q < NG1; q += 4) {
-            const int pix = q * 16 + j, pp = pix < RR * RC ? pix : RR * RC - 1;
+            const int pix = q * 16 + j, pp = pix < R12 * RC ? pix : R12 * RC - 1;
             const int r = pp / RC, c = pp - r * RC;
             const bf16x8 B = conv_in_b(sxs, XC2, r, c, lane);
             const floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
             const floatx4 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[0], B, z, 0, 0, 0);
             const floatx4 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A1[1], B, z, 0, 0, 0);
-            if (pix < RR * RC) {
+            if (pix < R12 * RC) {
                 const int gy = y0 + r - 1, gx = x0 + c - 1;
                 const bool inside = gy >= 0 && gy < H && gx >= 0 && gx < W;
                 put_grown(sx, r, c, 4 * g, acc0, sw + 9 * CH, inside);
@@ -397,8 +402,8 @@ __global__ void __launch_bounds__(256) k_conv12(const float *__restrict__ x, flo
         }
         __syncthreads();
 #pragma unroll 1
-        for (int q = 0; q < 4; q++) {
-            const int r = wave * 2 + (q >> 1), cb = (q & 1) * 16;
+        for (int q = 0; q < TH12 / 2; q++) {
+            const int r = wave * (TH12 / 4) + (q >> 1), cb = (q & 1) * 16;
             floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int t = 0; t < 9; t++) {
@@ -1324,7 +1329,10 @@ int gr_cnn_forward(const float *params, const unsigned short *packed, const floa
     // activation): the first layer fused into the second
     const bool fused = a3 == a1 && a4 == a2;
     if (fused) {
-        hipLaunchKernelGGL(k_conv12, dim3(nconv), dim3(256), 0, s, x, scale, params, pk, a2, H, W, ntiles, tm);
+        const TileMap tm12{tm.tiles_x, (H + TH12 - 1) / TH12};
+        const int nt12 = tm12.tiles_x * tm12.tiles_y * n;
+        hipLaunchKernelGGL(k_conv12, dim3(nt12 < MAX_CONV_BLOCKS ? nt12 : MAX_CONV_BLOCKS), dim3(256), 0, s, x, scale,
+                           params, pk, a2, H, W, nt12, tm12);
         conv32(a2, pk + PACK_LAYER, params + OFF_B3, a3);
         conv32(a3, pk + 2L * PACK_LAYER, params + OFF_B4, a4);
     } else {
