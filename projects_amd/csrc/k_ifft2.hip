@@ -148,8 +148,9 @@ struct WindowCol {
 
 // Column group of a workgroup.  Workgroups are dealt to the 8 XCDs round
 // robin, each XCD with its own L2; numbering the groups so that x, x+8, x+16,
-// ... (one XCD) take neighbouring column groups lets the groups that share a
-// 128-byte line of Z fetch it through one L2 instead of four.
+// ... (one XCD) take neighbouring column groups keeps neighbouring blocks of
+// the column-blocked Z on one XCD (narrower groups than four columns share
+// each block's lines).
 __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
     if (ngroups % 8u) return x;
     return (long)(x % 8u) * (ngroups / 8u) + x / 8u;
@@ -157,11 +158,10 @@ __device__ __forceinline__ long column_group(unsigned x, unsigned ngroups) {
 
 // CW > 0: CW neighbouring columns per workgroup, transformed side by side by
 // CW groups of nthr/CW threads (the workgroup's barriers are shared, so the
-// groups run the same passes in lockstep); the fill reads the CW columns of a
-// row from one cache line.  The column gathers -- one line per row and column
-// group, strided by a row of Z -- are what bounds this kernel, so wider groups
-// fetch fewer lines for the same bytes.  CW = 0: A.cw columns one after
-// another on all threads (small M, where CW * M is small).
+// groups run the same passes in lockstep); the fill reads the CW columns of
+// a row as neighbouring bytes of the column-blocked Z (four columns: one
+// contiguous block).  CW = 0: A.cw columns one after another on all threads
+// (small M, where CW * M is small).
 template <int MAXB, int CW>
 __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
     extern __shared__ float2 cols[];  // cw columns of gr_lds_len(M) slots
@@ -263,7 +263,8 @@ __global__ void __launch_bounds__(1024) k_ifft_cols(ColsArgs A) {
 // Persistent column transform for M = 4096 with 4 columns per item and
 // radix-16 passes (the plain 2048-detector geometry): the columns and the
 // 32 KiB of twiddles fill the CU's 160 KiB of LDS, so one workgroup of 1024
-// threads runs per CU and loops over (pair, column group) items.  Each thread
+// threads runs per CU and loops over (pair, column group) items, each one
+// contiguous 128 KB block of the column-blocked Z.  Each thread
 // loads rows j, j + 256, ..., j + 3840 of one column -- exactly the inputs of
 // the first radix-16 butterfly j -- so that pass runs from the registers that
 // received the loads; the next item's loads are issued into the same
