@@ -251,6 +251,34 @@ def test_pair_grouping_at_4096_is_bitwise_neutral(grid_op):
         p.close()
 
 
+def test_persistent_transforms_with_ragged_window_match_oracle():
+    """A 2047-detector half-padded geometry (pdim = M = 4096) with a 2047-wide
+    window: the persistent row and column transforms run with a padded last
+    column block of Z (2047 is not a multiple of 4).  The images match the
+    FP32 oracle's padded-and-cropped reconstruction to FP32 rounding, and the
+    persistent column transform is bitwise the one-column-per-workgroup
+    form."""
+    rng = np.random.default_rng(2047)
+    n_ang, n_det, center = 24, 2047, 1023.4
+    sino = rng.standard_normal((2, n_ang, n_det)).astype(np.float32)
+    p = gr.Plan(n_ang=n_ang, n_det=n_det, center=center, pad="half", crop=(n_det // 2, n_det))
+    assert p.info["M"] == 4096 and p.out_n == n_det
+    out = p.recon(torch.from_numpy(sino).cuda())
+    torch.cuda.synchronize()
+    p.set_cols_per_block(1)
+    out1 = p.recon(torch.from_numpy(sino).cuda())
+    torch.cuda.synchronize()
+    assert torch.equal(out, out1)
+    ref = O.recon_batch_f32(O.Geometry(n_ang=n_ang, n_det=n_det, center=center), sino, True, n_det // 2, n_det,
+                            threads=8)
+    o = out.cpu().numpy()
+    for k in range(2):
+        err = float(np.linalg.norm(o[k].astype(np.float64) - ref[k]) / np.linalg.norm(ref[k].astype(np.float64)))
+        print(f"slice {k}: rel-L2 vs oracle {err:.2e}", flush=True)
+        assert err < 1e-5, (k, err)
+    p.close()
+
+
 def test_column_widths_are_bitwise_neutral():
     """The column transform's workgroup width (columns side by side, or one
     after another) changes only the launch shape: every width a plan accepts
