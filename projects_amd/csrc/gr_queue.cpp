@@ -156,12 +156,44 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     }
     hipStream_t s0 = q->streams[0];
 
+    // size classes (pdim, M, output window); each is tuned on its largest dataset
+    std::map<std::tuple<long, long, long>, std::vector<int>> classes;
+    for (int d = 0; d < n_datasets; d++) {
+        const GrGeom &g = q->plans[d]->g;
+        classes[std::make_tuple(g.pdim, g.M, g.out_n)].push_back(d);
+    }
+    auto tuned_dataset = [&](const std::vector<int> &members) {
+        int d0 = members[0];
+        for (int d : members)
+            if (q->ds[d].n_slices > q->ds[d0].n_slices) d0 = d;
+        return d0;
+    };
+
     // gather operators: a queue is built once and replayed, so a dataset's
-    // operator pays off from the first replay on.  Within a budget -- three
-    // quarters of the device memory left after a generous estimate of the
-    // workspaces --
-    // datasets are served in order of slices (the operator's benefit per byte
+    // operator pays off from the first replay on.  The dataset each class is
+    // tuned on gets its operator first (the tuner times the kernels the class
+    // will run), within three quarters of the device memory left after a
+    // generous estimate of the workspaces; the others get theirs after
+    // tuning, when the workspaces' sizes are known, within seven eighths of
+    // what those leave -- in order of slices (the operator's benefit per byte
     // grows with the pairs it serves), then of size.
+    std::vector<int> op_order(n_datasets);
+    std::iota(op_order.begin(), op_order.end(), 0);
+    std::stable_sort(op_order.begin(), op_order.end(), [&](int a, int b) {
+        if (q->ds[a].n_slices != q->ds[b].n_slices) return q->ds[a].n_slices > q->ds[b].n_slices;
+        const GrGeom &ga = q->plans[a]->g, &gb = q->plans[b]->g;
+        return (long)q->plans[a]->prm.n_ang * ga.half > (long)q->plans[b]->prm.n_ang * gb.half;
+    });
+    auto build_operators = [&](const std::vector<char> &want, size_t budget) {
+        for (int d : op_order) {
+            if (!want[d] || !queue_op[d] || q->plans[d]->gop_state != 0 || budget == 0) continue;
+            const int brc = gr_plan_build_operator(q->plans[d], s0, false, budget);
+            if (brc) return brc;
+            const size_t b = q->plans[d]->gop_state == 1 ? q->plans[d]->gop_bytes : 0;
+            budget = b < budget ? budget - b : 0;
+        }
+        return (int)GR_OK;
+    };
     {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
@@ -169,33 +201,12 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         for (int d = 0; d < n_datasets; d++)
             ws_est = std::max(ws_est, gr_plan_workspace_bytes(q->plans[d], std::min((q->ds[d].n_slices + 1) / 2, 8)));
         ws_est *= (size_t)(n_streams + 1);
-        size_t budget = free_b > ws_est ? (free_b - ws_est) / 4 * 3 : 0;
-        std::vector<int> order(n_datasets);
-        std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
-            if (q->ds[a].n_slices != q->ds[b].n_slices) return q->ds[a].n_slices > q->ds[b].n_slices;
-            const GrGeom &ga = q->plans[a]->g, &gb = q->plans[b]->g;
-            return (long)q->plans[a]->prm.n_ang * ga.half > (long)q->plans[b]->prm.n_ang * gb.half;
-        });
-        for (int d : order) {
-            if (!queue_op[d] || budget == 0) continue;
-            QCK(gr_plan_build_operator(q->plans[d], s0, false, budget));
-            const size_t b = q->plans[d]->gop_bytes;
-            budget = b < budget ? budget - b : 0;
-        }
-        for (int d = 0; d < n_datasets; d++) {
-            if (q->plans[d]->gop_state != 1) continue;
-            q->info.datasets_with_operator++;
-            q->info.operator_bytes += q->plans[d]->gop_bytes;
-        }
+        std::vector<char> rep(n_datasets, 0);
+        for (auto &kv : classes) rep[tuned_dataset(kv.second)] = 1;
+        QCK(build_operators(rep, free_b > ws_est ? (free_b - ws_est) / 4 * 3 : 0));
     }
 
-    // size classes and the autotuner
-    std::map<std::tuple<long, long, long>, std::vector<int>> classes;
-    for (int d = 0; d < n_datasets; d++) {
-        const GrGeom &g = q->plans[d]->g;
-        classes[std::make_tuple(g.pdim, g.M, g.out_n)].push_back(d);
-    }
+    // the autotuner
     const int kPairChoices[] = {1, 2, 4, 8};
     size_t tune_bytes = 0;
     for (int d = 0; d < n_datasets; d++) {
@@ -210,9 +221,7 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
     q->cost_ms.assign(n_datasets, 0.0);
     int tuned = 0;
     for (auto &kv : classes) {
-        int d0 = kv.second[0];  // tune on the class's largest dataset
-        for (int d : kv.second)
-            if (q->ds[d].n_slices > q->ds[d0].n_slices) d0 = d;
+        const int d0 = tuned_dataset(kv.second);
         gr_plan *p = q->plans[d0];
         const int pairs = (q->ds[d0].n_slices + 1) / 2;
         int best_mp = std::min(pairs, 8), best_cw = 0, best_radix = 16, best_wpb = 4;
@@ -288,6 +297,23 @@ int gr_queue_create(const gr_dataset *datasets, int n_datasets, int device, int 
         size_t pass_bytes = 0;
         for (int d = 0; d < n_datasets; d++)
             pass_bytes = std::max(pass_bytes, gr_plan_workspace_bytes(q->plans[d], q->plans[d]->max_pairs));
+        // the other datasets' operators: the stream workspaces allocated below
+        // hold at most n_streams times the largest tuned workspace, and the
+        // tuning buffer is freed before them
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        const size_t avail = free_b + tune_bytes, need = (size_t)n_streams * pass_bytes;
+        rc = build_operators(queue_op, avail > need ? (avail - need) / 8 * 7 : 0);
+        if (rc) {
+            (void)hipFree(tune_ws);
+            free_queue(q);
+            return rc;
+        }
+        for (int d = 0; d < n_datasets; d++) {
+            if (q->plans[d]->gop_state != 1) continue;
+            q->info.datasets_with_operator++;
+            q->info.operator_bytes += q->plans[d]->gop_bytes;
+        }
         if (pass_bytes > tune_bytes) {
             (void)hipFree(tune_ws);
             tune_ws = nullptr;
