@@ -88,6 +88,18 @@ __device__ __forceinline__ long spec_group_offset(const GridArgs &A, int pair) {
     return (long)(pair / kSpecGroup) * A.n_ang * A.half * kSpecGroup + pair % kSpecGroup;
 }
 
+// Element i of a table that no kernel writes, read through the constant
+// address space: at a wave-uniform index that is a scalar load, which does not
+// wait behind the wave's vector loads in the vector memory counter.
+template <class T>
+__device__ __forceinline__ T table_load(const T *p, long i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return ((const __attribute__((address_space(4))) T *)p)[i];
+#else
+    return p[i];  // the host pass of a device function: never called
+#endif
+}
+
 // s + c += x with s the float sum and c the carried rounding errors.
 __device__ __forceinline__ void two_sum_add(float &s, float &c, float x) {
     const float t = s + x;
@@ -189,7 +201,7 @@ __device__ __forceinline__ void walk_tile(const GridArgs &A, int tx, int ty, boo
     for (int side = 0; side < 2; side++) {
         const int k0 = side ? r2 : r0, k1 = side ? r3 : r1;
         for (int k = k0; k < k1; k++) {
-            const float4 cs = A.ang[k];
+            const float4 cs = table_load(A.ang, k);
             // does the line pass through the tile grown by the box half-width?
             if (fabsf(cu * cs.y - cv * cs.x) > reach * (fabsf(cs.x) + fabsf(cs.y)) + 0.05f) continue;
             int td0, td1, tm0, tm1;
@@ -201,7 +213,7 @@ __device__ __forceinline__ void walk_tile(const GridArgs &A, int tx, int ty, boo
                 jd1 = jd0 - 1;
                 jm1 = jm0 - 1;
             }
-            const long row_off = (long)A.aord[k] * A.half;
+            const long row_off = (long)table_load(A.aord, k) * A.half;
             for (int c0 = td0; c0 <= td1; c0 += 64)
                 on_chunk(0, row_off, c0, min(64, td1 - c0 + 1), max(jd0, c0), min(min(jd1, td1), c0 + 63), cs.x,
                          cs.y);
@@ -244,10 +256,13 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     }
     __syncthreads();
 
+    // the tile slot as a wave-uniform value: the tile, its angle window and
+    // each line's angle record then come through scalar loads, which do not
+    // wait behind the spectra loads in the vector memory counter
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ti = sector_slot(A, blockIdx.x, blockDim.x >> 6, wave, 0);
+    const int ti = __builtin_amdgcn_readfirstlane(sector_slot(A, blockIdx.x, blockDim.x >> 6, wave, 0));
     if (ti < 0) return;
-    const unsigned tt = A.tiles[ti];
+    const unsigned tt = table_load(A.tiles, ti);
     if (tt == kGridNoTile) return;
     float2(*stage)[P] = reinterpret_cast<float2(*)[P]>(smem + A.lds_wtbl) + wave * 64;
     const int pair0 = A.pair_base + blockIdx.y * P;
@@ -337,9 +352,9 @@ __global__ void __launch_bounds__(256) k_gop_build(GridArgs A, unsigned *tile_n,
                                                    const unsigned *tile_off, uint2 *gop, unsigned *tile_g,
                                                    const uint4 *hmeta, uint2 *hseg, int gps) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
+    const int ti = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wave);  // scalar table loads
     if (ti >= A.ntiles) return;
-    const unsigned tt = A.tiles[ti];
+    const unsigned tt = table_load(A.tiles, ti);
     if (tt == kGridNoTile) {  // padding slot: no rows
         if (MODE == 0 && lane == 0) tile_n[ti] = tile_na[ti] = tile_g[ti] = 0;
         return;
@@ -463,9 +478,9 @@ template <int MODE>
 __global__ void __launch_bounds__(256) k_sop_build(GridArgs A, int n_heavy, uint4 *tile, uint2 *ch, unsigned *keys,
                                                    unsigned *ent) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ti = blockIdx.x * (blockDim.x >> 6) + wave;
+    const int ti = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + wave);  // scalar table loads
     if (ti >= A.ntiles) return;
-    const unsigned tt = A.tiles[ti];
+    const unsigned tt = table_load(A.tiles, ti);
     if (tt == kGridNoTile || ti % A.sector_len < n_heavy) {
         if (MODE == 0 && lane == 0) tile[ti] = make_uint4(0u, 0u, 0u, 0u);
         return;
