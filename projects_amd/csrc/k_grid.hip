@@ -147,39 +147,31 @@ __device__ __forceinline__ void store_pairs(float2 *dst, const float2 (&v)[P]) {
     }
 }
 
-// Stage samples c0 .. c0+nj-1 of the sample row at row_off (one lane per
-// sample, all P pairs of the group), then add the taps of samples [j0, j1]
-// into cell (qa, qb).  In a partial group the spare slots hold other data:
-// their accumulators are never stored.
+// Add the taps of samples [j0, j1] into cell (qa, qb) from one chunk of a
+// sample row staged in the wave's LDS buffer stage (slot l: the P pairs'
+// values of sample c0 + l).  In a partial group the spare slots hold other
+// data: their accumulators are never stored.
 template <int ACCUM, int P>
-__device__ __forceinline__ void chunk(const float2 *spec, long row_off, int c0, int nj, int j0, int j1, float C,
-                                      float S, float qa, float qb, const GrGatherGeom &g, const float *wtbl,
-                                      const float *dwtbl, float2 (*stage)[P], int lane, float4 (&acc)[P],
-                                      float2 (&part)[P]) {
-    if (lane < nj) {
-        float2 v[P];
-        load_pairs<P>(spec + (row_off + c0 + lane) * kSpecGroup, v);
-        store_pairs<P>(stage[lane], v);
-    }
-    __builtin_amdgcn_wave_barrier();
+__device__ __forceinline__ void chunk_taps(const float2 (*stage)[P], int c0, int j0, int j1, float C, float S,
+                                           float qa, float qb, const GrGatherGeom &g, const float *wtbl,
+                                           const float *dwtbl, float4 (&acc)[P], float2 (&part)[P]) {
     for (int j = j0; j <= j1; j++) {
         float w;
         if (dwtbl ? gr_tap_interp(j, C, S, qa, qb, g, wtbl, dwtbl, w) : gr_tap(j, C, S, qa, qb, g, wtbl, w)) {
-            float2 v[P];
-            load_pairs<P>(stage[j - c0], v);
+            float2 u[P];
+            load_pairs<P>(stage[j - c0], u);
 #pragma unroll
             for (int p = 0; p < P; p++) {
                 if (ACCUM == GR_ACCUM_COMPENSATED) {
-                    part[p].x = __builtin_fmaf(w, v[p].x, part[p].x);
-                    part[p].y = __builtin_fmaf(w, v[p].y, part[p].y);
+                    part[p].x = __builtin_fmaf(w, u[p].x, part[p].x);
+                    part[p].y = __builtin_fmaf(w, u[p].y, part[p].y);
                 } else {
-                    acc[p].x += w * v[p].x;
-                    acc[p].z += w * v[p].y;
+                    acc[p].x += w * u[p].x;
+                    acc[p].z += w * u[p].y;
                 }
             }
         }
     }
-    __builtin_amdgcn_wave_barrier();
 }
 
 // Walk the sample lines that can reach the wave's tile, in the kernel's fixed
@@ -264,7 +256,7 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     if (ti < 0) return;
     const unsigned tt = table_load(A.tiles, ti);
     if (tt == kGridNoTile) return;
-    float2(*stage)[P] = reinterpret_cast<float2(*)[P]>(smem + A.lds_wtbl) + wave * 64;
+    float2(*stage)[P] = reinterpret_cast<float2(*)[P]>(smem + A.lds_wtbl) + wave * 128;  // two buffers of 64 slots
     const int pair0 = A.pair_base + blockIdx.y * P;
     const int nvalid = min(P, A.pair_end - pair0);
     const int tx = (int)(tt & 0xffffu), ty = (int)(tt >> 16);
@@ -285,18 +277,45 @@ __global__ void __launch_bounds__(256) k_grid(GridArgs A) {
     int lines = 0;  // sample lines in the open partial
     const long group = spec_group_offset(A, pair0);
     const float2 *base1 = A.spec1 + group, *base2 = A.spec2 + group;
+    // one chunk in flight: a chunk's samples are loaded when walk_tile hands
+    // it over, the taps of the chunk before are added from the other LDS
+    // buffer while the loads travel, and only then do the new values go to
+    // LDS -- the loaded registers never live past their chunk.  Every lane
+    // loads (past the chunk's end it repeats its last sample, never read), so
+    // no load sits in a branch.  The taps and the merges after each line
+    // keep their order.
+    bool have = false, p_eol = false;
+    int cur = 0, p_mirror = 0, p_c0 = 0, p_j0 = 0, p_j1 = -1;
+    float p_C = 0.0f, p_S = 0.0f;
+    auto add_pending = [&]() {
+        chunk_taps<ACCUM, P>(stage + cur * 64, p_c0, p_j0, p_j1, p_C, p_S, p_mirror ? ma : pa, p_mirror ? mb : pb,
+                             A.g, s_wtbl, s_dwtbl, acc, part);
+        if (p_eol && ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
+            merge<P>(acc, part);
+            lines = 0;
+        }
+    };
     walk_tile(
         A, tx, ty, valid, pa - M2, pb - M2,
         [&](int mirror, long row_off, int c0, int nj, int j0, int j1, float C, float S) {
-            chunk<ACCUM, P>(mirror ? base2 : base1, row_off, c0, nj, j0, j1, C, S, mirror ? ma : pa,
-                            mirror ? mb : pb, A.g, s_wtbl, s_dwtbl, stage, lane, acc, part);
+            float2 v[P];
+            load_pairs<P>((mirror ? base2 : base1) + (row_off + c0 + min(lane, nj - 1)) * kSpecGroup, v);
+            if (have) add_pending();
+            cur ^= 1;
+            __builtin_amdgcn_wave_barrier();
+            store_pairs<P>(stage[cur * 64 + lane], v);
+            __builtin_amdgcn_wave_barrier();
+            have = true;
+            p_eol = false;
+            p_mirror = mirror;
+            p_c0 = c0;
+            p_j0 = j0;
+            p_j1 = j1;
+            p_C = C;
+            p_S = S;
         },
-        [&]() {
-            if (ACCUM == GR_ACCUM_COMPENSATED && ++lines == kMergeAngles) {
-                merge<P>(acc, part);
-                lines = 0;
-            }
-        });
+        [&]() { p_eol = true; });  // the pending chunk ends its line
+    if (have) add_pending();
     if (ACCUM == GR_ACCUM_COMPENSATED && lines) merge<P>(acc, part);
     store_cells<ACCUM, P>(A, pair0, nvalid, a, b, acc);
 }
@@ -1046,7 +1065,7 @@ hipError_t launch(const GridArgs &A, const GopArgs *O, int groups, size_t lds_ba
             hipLaunchKernelGGL((k_grid_op<ACCUM, P, true>), dim3(sblocks, groups), dim3(256), lds, s, A, o);
         return hipGetLastError();
     }
-    const size_t lds = lds_base + sizeof(float2) * 64 * wpb * P;
+    const size_t lds = lds_base + sizeof(float2) * 128 * wpb * P;  // two staging buffers per wave
     hipError_t e = gr_allow_lds(k_grid<ACCUM, P>, lds);
     if (e != hipSuccess) return e;
     const int blocks = kGridSectors * ((A.sector_len + wpb - 1) / wpb);
