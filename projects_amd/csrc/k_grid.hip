@@ -762,9 +762,9 @@ __device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs 
     auto fetch = [&](int slot) {  // slot < sector_len
         const int ti = first + slot;
         Next n;
-        n.tt = A.tiles[ti];
-        n.td = O.sop_tile[ti];
-        n.h = O.sop_ch[ti];
+        n.tt = table_load(A.tiles, ti);  // wave-uniform: scalar loads
+        n.td = table_load(O.sop_tile, ti);
+        n.h = table_load(O.sop_ch, ti);
         n.k0 = O.sop_keys[(unsigned long)ti * kSopSlots + lane];
         n.k1 = O.sop_keys[(unsigned long)ti * kSopSlots + 64 + lane];
         return n;
@@ -812,7 +812,7 @@ __device__ __forceinline__ void grid_sop_light(const GridArgs &A, const GopArgs 
             if (ns > 64u) sop_stage<P>(k1, base1, base2, stg, 64);
             // the next chunk (or this one again past the last)
             const unsigned cn = c + 1 < nch ? extra + c : (c ? extra + c - 1 : cur_ti);
-            const uint2 hn = O.sop_ch[cn];
+            const uint2 hn = table_load(O.sop_ch, cn);
             const unsigned kn0 = O.sop_keys[(unsigned long)cn * kSopSlots + lane];
             const unsigned kn1 = O.sop_keys[(unsigned long)cn * kSopSlots + 64 + lane];
             // rows come in whole batches of kSopRowBatch: no per-row guard,
